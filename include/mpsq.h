@@ -1,0 +1,198 @@
+/*
+ * mpsq.h -- C ABI of the MI355X S(Q,w) engine (libmpsq.so).
+ *
+ * Two layers:
+ *   1. A FINUFFT-shaped type-1 NUFFT (plan / setpts / execute / destroy, FP64
+ *      and FP32 "f" variants, 1-3 dimensions, ntrans strength vectors), the
+ *      drop-in for the reference's calls into FINUFFT, plus Fortran-ABI
+ *      shims with the reference's by-reference argument list.
+ *   2. A device-resident S(Q,w) engine: per-frame, per-species type-1 NUFFT
+ *      of staged trajectory positions, species combine, and the Hann-windowed
+ *      time-FFT reduction to S(Q) and S(Q,w).
+ *
+ * Return codes follow FINUFFT: 0 = ok, 1 = warning (eps below the precision
+ * floor of the plan; the result is still computed at the floor), >= 2 =
+ * error, in which case every output array is zero-filled.  No entry point
+ * prints, exits or aborts.
+ *
+ * Complex arrays are interleaved (re, im) pairs: double[2] per value for the
+ * FP64 API, float[2] for the FP32 API.  Mode order is CMCL: k1 fastest,
+ * k_d = -N_d/2 .. (N_d-1)/2.
+ *
+ * Citations are to the jkulda/MP_tools tree.
+ */
+#ifndef MPSQ_H
+#define MPSQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MPSQ_OK = 0,
+    MPSQ_WARN_EPS_TOO_SMALL = 1,
+    MPSQ_ERR_BAD_ARG = 2,
+    MPSQ_ERR_NONFINITE_POINT = 3,
+    MPSQ_ERR_SIZE = 4,         /* a fine-grid dimension exceeds the LDS tile */
+    MPSQ_ERR_DEVICE = 5,       /* HIP runtime / allocation failure */
+    MPSQ_ERR_NOT_SET = 6       /* execute before setpts */
+};
+
+enum { MPSQ_FP32 = 0, MPSQ_FP64 = 1 };
+
+typedef struct mpsq_opts {
+    int device;        /* HIP device ordinal (default 0) */
+    void *stream;      /* hipStream_t used by every launch of the plan, NULL = own stream */
+    int tile_by;       /* spread tile rows along y (0 = tuned/heuristic) */
+    int tile_bz;       /* spread tile rows along z (0 = tuned/heuristic) */
+    int sort_chunk;    /* points per sort chunk (0 = heuristic) */
+    int debug;         /* reserved, 0 */
+} mpsq_opts;
+
+/* Defaults: device 0, own stream, heuristic tiles. */
+void mpsq_default_opts(mpsq_opts *o);
+
+/* ---------------- FINUFFT-shaped plan API, FP64 ---------------- */
+typedef struct mpsq_plan_s *mpsq_plan;
+
+/* Replaces FINUFFT's finufft_makeplan (type 1 only); the reference calls the
+ * simple interface finufft2d1 at mp_sqom56.f90:1148 and mp_sql56.f90:1129,
+ * which is this plan's makeplan + setpts + execute + destroy. */
+int mpsq_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntrans, double eps,
+                  mpsq_plan *plan, const mpsq_opts *opts);
+/* Copies M points (host arrays; y/z ignored below dim 2/3) to the device,
+ * folds them in FP64 and bin-sorts them.  Any finite coordinate is accepted
+ * and folded periodically with period 2 pi. */
+int mpsq_setpts(mpsq_plan plan, int64_t M, const double *x, const double *y, const double *z);
+/* c: ntrans*M complex strengths (host), f: ntrans*N1*N2*N3 complex modes (host). */
+int mpsq_execute(mpsq_plan plan, const double *c, double *f);
+/* Device-resident variants: every pointer is device memory, work is queued on
+ * the plan's stream and the call returns without synchronising. */
+int mpsq_setpts_dev(mpsq_plan plan, int64_t M, const double *d_x, const double *d_y, const double *d_z);
+int mpsq_execute_dev(mpsq_plan plan, const double *d_c, double *d_f);
+int mpsq_destroy(mpsq_plan plan);
+/* Bin-sort permutation of the last setpts (perm[k] = caller index of the
+ * k-th point in bin order), copied to host memory of M int32. */
+int mpsq_get_sort_perm(mpsq_plan plan, int32_t *perm);
+/* Fine grid size, kernel width and tile geometry chosen by makeplan:
+ * out[0..2] = n1,n2,n3, out[3] = w, out[4] = by, out[5] = bz, out[6] = BY,
+ * out[7] = BZ. */
+int mpsq_plan_info(mpsq_plan plan, int64_t *out);
+
+/* ---------------- FINUFFT-shaped plan API, FP32 ---------------- */
+typedef struct mpsq_planf_s *mpsq_planf;
+int mpsqf_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntrans, double eps,
+                   mpsq_planf *plan, const mpsq_opts *opts);
+int mpsqf_setpts(mpsq_planf plan, int64_t M, const float *x, const float *y, const float *z);
+int mpsqf_execute(mpsq_planf plan, const float *c, float *f);
+int mpsqf_setpts_dev(mpsq_planf plan, int64_t M, const float *d_x, const float *d_y, const float *d_z);
+int mpsqf_execute_dev(mpsq_planf plan, const float *d_c, float *d_f);
+int mpsqf_destroy(mpsq_planf plan);
+int mpsqf_get_sort_perm(mpsq_planf plan, int32_t *perm);
+int mpsqf_plan_info(mpsq_planf plan, int64_t *out);
+
+/* ---------------- simple interface (FINUFFT's finufft{1,2,3}d1) ---------------- */
+int mpsq_nufft1d1(int64_t nj, const double *xj, const double *cj, int iflag, double eps, int64_t ms,
+                  double *fk, const mpsq_opts *opts);
+int mpsq_nufft2d1(int64_t nj, const double *xj, const double *yj, const double *cj, int iflag, double eps,
+                  int64_t ms, int64_t mt, double *fk, const mpsq_opts *opts);
+int mpsq_nufft3d1(int64_t nj, const double *xj, const double *yj, const double *zj, const double *cj, int iflag,
+                  double eps, int64_t ms, int64_t mt, int64_t mu, double *fk, const mpsq_opts *opts);
+
+/* ---------------- Fortran ABI (gfortran/flang name mangling, by reference) ----------------
+ * Exactly the argument list of the reference's implicit-interface call
+ *   call finufft2d1(n_fft,xf,yf,cf,iflag,eps_fft,n_qx8,n_qy8,ampl_tot,nul_opt,ier)
+ * (mp_sqom56.f90:1148; declarations :94-98,106,195-196; mp_sql56.f90:1129).
+ * opts may be a null pointer (the reference passes an unallocated allocatable);
+ * when non-null it points to an mpsq_opts.  fk is always fully overwritten. */
+void finufft1d1_(const int64_t *nj, const double *xj, const double *cj, const int *iflag, const double *eps,
+                 const int64_t *ms, double *fk, const void *opts, int *ier);
+void finufft2d1_(const int64_t *nj, const double *xj, const double *yj, const double *cj, const int *iflag,
+                 const double *eps, const int64_t *ms, const int64_t *mt, double *fk, const void *opts, int *ier);
+void finufft3d1_(const int64_t *nj, const double *xj, const double *yj, const double *zj, const double *cj,
+                 const int *iflag, const double *eps, const int64_t *ms, const int64_t *mt, const int64_t *mu,
+                 double *fk, const void *opts, int *ier);
+/* Fortran-callable plan handle wrappers (handle = int64). */
+void mpsq_makeplan_(const int *type, const int *dim, const int64_t *n_modes, const int *iflag, const int *ntrans,
+                    const double *eps, int64_t *handle, const void *opts, int *ier);
+void mpsq_setpts_(const int64_t *handle, const int64_t *M, const double *x, const double *y, const double *z,
+                  int *ier);
+void mpsq_execute_(const int64_t *handle, const double *c, double *f, int *ier);
+void mpsq_destroy_(const int64_t *handle, int *ier);
+
+/* ---------------- S(Q,w) engine ----------------
+ * Trajectory frames are device-resident FP32 records (x, y, z, charge) in the
+ * MP_TOOLS binary layout (at_pos(4,n_tot), code/docs/1_mp_data_doc_154.txt:
+ * 100-104), frame-major: pos[f][atom][4], atoms species-major (CELL record
+ * order mp_lbin56.f90:1204; BULK species blocks mp_lbin56.f90:1336-1347).
+ * Per frame and active species the engine stages the NUFFT coordinates and
+ * strengths (mp_sqom56.f90:1104-1137), runs a type-1 NUFFT
+ * (mp_sqom56.f90:1148), and combines the species with their weights
+ * (mp_sqom56.f90:1338-1356) into C(t,Q), layout [T][NQ]. */
+typedef struct mpsq_sqw_s *mpsq_sqw;
+
+typedef struct mpsq_sqw_spec {
+    int dim;                  /* 2 (mp_sqom Q plane) or 3 */
+    int64_t n_modes[3];       /* N1, N2, N3 (unused = 1) */
+    double eps;               /* NUFFT tolerance (reference: 1e-6, mp_sqom56.f90:195) */
+    int precision;            /* MPSQ_FP32 or MPSQ_FP64 */
+    int iflag;                /* sign of the exponent (reference: -1, :196) */
+    int64_t n_tot;            /* atoms per frame */
+    int nsp;                  /* species */
+    const int64_t *sp_count;  /* [nsp] atoms of each species (species-major) */
+    const int32_t *mask;      /* [nsp] 0/1 or NULL = all (at_mask, :1345) */
+    const double *w_scalar;   /* [nsp] weights (unit / 0.1*b_coh, :1347-1350) or NULL = 1 */
+    const float *w_table;     /* [nsp][NQ] per-Q weights (x_ffq, :1351-1352) host, or NULL */
+    int combine;              /* 1: C(t,Q) = sum_j w_j F_j; 0: per-species F_j, layout [T][nact][NQ] */
+    double centre[3];         /* at_pos_centre */
+    double axes[9];           /* unit plane vectors e1, e2, e3 (rows) (:799-807) */
+    double tpe[3];            /* 2 pi / (e_d . (pmax - pmin)) (:809-811) */
+    double q0[3];             /* q_center: strength phase exp(-i 2 pi q0 . r) (:1117) */
+    int window;               /* spatial Hann window (j_wind, :1118, :1167) */
+    int vacancy_check;        /* drop atoms at (0,0,0) (:1106) */
+    int select_mode;          /* 0 none, 1 CELL window on at_ind (:1107-1113), 2 BULK box (:1123-1132) */
+    int32_t n_row[3];
+    float pmin[3], pmax[3];   /* at_pos_min / at_pos_max */
+    float margin;             /* BULK margin (eps_x = 0.1 for one box, else 1) */
+    const int32_t *at_ind;    /* host [n_tot][4] cell indices for select_mode 1, or NULL */
+    int batch_frames;         /* frames per launch batch (0 = auto) */
+    mpsq_opts opts;
+} mpsq_sqw_spec;
+
+void mpsq_sqw_default_spec(mpsq_sqw_spec *s);
+int mpsq_sqw_create(const mpsq_sqw_spec *spec, mpsq_sqw *eng);
+/* d_pos: device float[T][n_tot][4]; d_amp: device complex [T][NQ] (combine)
+ * or [T][nact][NQ], FP32 or FP64 per the spec's precision. */
+int mpsq_sqw_amplitudes(mpsq_sqw eng, const float *d_pos, int64_t T, void *d_amp);
+
+typedef struct mpsq_reduce_spec {
+    int64_t n_int;            /* window length (frames); n_frame = T - n_int + 1 windows, hop 1 */
+    int win_kind;             /* 0: 0.5(1-cos(2 pi i/(n_int-1))) (:941-948); 1: 0.5(1-cos(2 pi (i+1)/(n_int+1))) (:2429-2431) */
+    double dt;                /* frame step t_step (ps) */
+    double nat;               /* N_at = sum of nsuper_r (:1574-1576) */
+    int64_t m_lo;             /* first frequency index (may be negative) */
+    int64_t n_f;              /* number of frequency bins written */
+    int subtract_mean;        /* 1: inelastic C' = C - <C>_t (:1358-1377) */
+    const float *d_qfac;      /* device [NQ] factor on C' (q2_norm when j_qsq != 1), or NULL */
+} mpsq_reduce_spec;
+
+/* d_S: device double[n_f][NQ]; d_Stot: device double[NQ] or NULL. */
+int mpsq_sqw_reduce(mpsq_sqw eng, const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S,
+                    double *d_Stot);
+int mpsq_sqw_destroy(mpsq_sqw eng);
+/* Synchronise the engine's stream. */
+int mpsq_sqw_sync(mpsq_sqw eng);
+/* Number of modes N1*N2*N3 and active species of an engine. */
+int64_t mpsq_sqw_nq(mpsq_sqw eng);
+int mpsq_sqw_nact(mpsq_sqw eng);
+
+/* Library version string. */
+const char *mpsq_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPSQ_H */

@@ -1,0 +1,720 @@
+// nufft_kernels.hip -- type-1 NUFFT kernels for gfx950.
+//
+// Pipeline for one batch of transforms (t = 0..ntr-1):
+//   k_fold / k_stage_traj   FP64 fold of coordinates into the periodic fine
+//                           grid: first kernel cell l (int) + offset d
+//                           (FP32 in the FP32 form), bin key, strength.
+//   k_sort_count            per-chunk bin histogram (LDS)
+//   k_scan_*                exclusive scan of the (transform, bin, chunk)
+//                           count matrix -> absolute destinations
+//   k_sort_scatter          stable scatter: ranks inside a chunk are taken
+//                           in point order, wave by wave, so the permutation
+//                           equals a stable counting sort by key
+//   k_spread_xfft           one workgroup per (y,z) pencil tile spanning the
+//                           whole x extent: exponential-of-semicircle
+//                           spreading of every point whose kernel touches
+//                           the tile into LDS (ds_add atomics), then the x
+//                           FFT of the tile's rows in LDS, truncation to the
+//                           kept k1 and deconvolution; only N1 of n1 x
+//                           frequencies ever reach HBM.
+//   k_colfft                the y and z passes: column FFTs of LDS tiles,
+//                           truncation, deconvolution; the last pass also
+//                           applies species weights and sums the species
+//                           of a frame (S(Q,w) combine, mp_sqom56.f90:1338-1356).
+#include "mpsq_device.h"
+#include "mpsq_kernels.h"
+
+namespace mpsq {
+
+// ---------------------------------------------------------------------------
+// FP64 fold (contraction off: matches the CPU restatement bit for bit)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void fold1(double xv, double s, double hw, int n, int &l, double &d, int &bad)
+{
+#pragma clang fp contract(off)
+    const double u = xv * s;
+    const double t = u - hw;
+    const double lf = ceil(t);
+    d = lf - u;
+    long long lw = 0;
+    if (isfinite(lf) && fabs(lf) < 9.0e18) lw = (long long)lf;
+    else bad = 1;
+    lw %= (long long)n;
+    if (lw < 0) lw += n;
+    l = (int)lw;
+}
+
+template <typename R>
+__device__ __forceinline__ int bin_key(const Geo &g, int ly, int lz)
+{
+    if (g.dim == 1) return 0;
+    if (g.dim == 2) return ly / g.BY;
+    return (lz / g.BZ) * g.nbY + ly / g.BY;
+}
+
+template <typename R, typename Tin>
+__global__ void k_fold(Geo g, int64_t M, const Tin *__restrict__ x, const Tin *__restrict__ y,
+                       const Tin *__restrict__ z, PtRec<R> *__restrict__ rec, int *__restrict__ key,
+                       int *__restrict__ bad_flag)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    int bad = 0;
+    int l[3] = {0, 0, 0};
+    double d[3] = {0.0, 0.0, 0.0};
+    const Tin *co[3] = {x, y, z};
+    const double sc[3] = {g.s1, g.s2, g.s3};
+    const int nn[3] = {g.n1, g.n2, g.n3};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k < g.dim) {
+            const double xv = (double)co[k][j];
+            if (!isfinite(xv)) bad = 1;
+            fold1(xv, sc[k], g.hw, nn[k], l[k], d[k], bad);
+        }
+    PtRec<R> r;
+    r.lx = l[0];
+    r.ly = l[1];
+    r.lz = l[2];
+    r.idx = (int)j;
+    r.dx = (R)d[0];
+    r.dy = (R)d[1];
+    r.dz = (R)d[2];
+    r.pad = (R)0;
+    rec[j] = r;
+    key[j] = bin_key<R>(g, l[1], l[2]);
+    if (bad) atomicOr(bad_flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// a2 + a3 staging of trajectory frames (mp_sqom56.f90:1104-1137):
+//   point p of batch = frame b (of the batch) * n_tot + atom a, a species-major
+//   transform t = b * nsp + species(a); transforms are contiguous in p.
+//   keep = not a vacancy (0,0,0) and inside the selection window
+//   x_d  = ((p0*e_d0 + p1*e_d1) + p2*e_d2) * tpe_d,  p = r - centre     (FP64)
+//   c    = keep ? exp(-i 2pi q0.r) * [prod_d (1+cos x_d)/2] : 0
+// Dropped atoms keep their slot with strength 0, which leaves every
+// amplitude unchanged and keeps the per-transform point counts fixed.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_stage_traj(Geo g, StageParams sp, int64_t npts, const float4 *__restrict__ pos,
+                             const int4 *__restrict__ at_ind, PtRec<R> *__restrict__ rec,
+                             int *__restrict__ key, typename cplx<R>::T *__restrict__ cstr,
+                             int *__restrict__ bad_flag)
+{
+#pragma clang fp contract(off)
+    using C = typename cplx<R>::T;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npts) return;
+    const int64_t a = p % sp.n_tot;
+    const float4 r = pos[p];
+    bool keep = true;
+    if (sp.vacancy_check) keep = (r.x != 0.0f) || (r.y != 0.0f) || (r.z != 0.0f);
+    if (sp.select_mode == 1 && at_ind) {  // CELL window on cell indices
+        const int4 ii = at_ind[a];
+        const int iv[3] = {ii.x, ii.y, ii.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float v = (float)(iv[k] - sp.n_row[k] / 2 - 1);
+            keep = keep && (v >= sp.pmin[k]) && (v <= sp.pmax[k]);
+        }
+    } else if (sp.select_mode == 2) {  // BULK box with margin, FP32 bounds
+        const float rv[3] = {r.x, r.y, r.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float lo = sp.pmin[k] - sp.margin, hi = sp.pmax[k] + sp.margin;
+            keep = keep && (rv[k] >= lo) && (rv[k] < hi);
+        }
+    }
+    const double r0 = (double)r.x, r1 = (double)r.y, r2 = (double)r.z;
+    const double p0 = r0 - sp.centre[0], p1 = r1 - sp.centre[1], p2 = r2 - sp.centre[2];
+    int bad = 0;
+    int l[3] = {0, 0, 0};
+    double d[3] = {0.0, 0.0, 0.0};
+    double xv[3] = {0.0, 0.0, 0.0};
+    const double sc[3] = {g.s1, g.s2, g.s3};
+    const int nn[3] = {g.n1, g.n2, g.n3};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (k < g.dim) {
+            const double e0 = p0 * sp.axes[3 * k];
+            const double e1 = p1 * sp.axes[3 * k + 1];
+            const double e2 = p2 * sp.axes[3 * k + 2];
+            const double dot = (e0 + e1) + e2;
+            xv[k] = dot * sp.tpe[k];
+            fold1(xv[k], sc[k], g.hw, nn[k], l[k], d[k], bad);
+        }
+    double cr = 0.0, ci = 0.0;
+    if (keep) {
+        const double ph = 6.283185307179586476925286766559 * ((sp.q0[0] * r0 + sp.q0[1] * r1) + sp.q0[2] * r2);
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        cr = cs;
+        ci = -sn;
+        if (sp.window) {
+            double wf = 1.0;
+            for (int k = 0; k < g.dim; ++k) wf *= 0.5 * (1.0 + cos(xv[k]));
+            cr *= wf;
+            ci *= wf;
+        }
+        if (sp.conj) ci = -ci;
+    }
+    PtRec<R> rr;
+    rr.lx = l[0];
+    rr.ly = l[1];
+    rr.lz = l[2];
+    rr.idx = (int)a;
+    rr.dx = (R)d[0];
+    rr.dy = (R)d[1];
+    rr.dz = (R)d[2];
+    rr.pad = (R)0;
+    rec[p] = rr;
+    key[p] = bin_key<R>(g, l[1], l[2]);
+    C cv;
+    cv.x = (R)cr;
+    cv.y = (R)ci;
+    cstr[p] = cv;
+    if (bad) atomicOr(bad_flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// stable bin sort
+// ---------------------------------------------------------------------------
+__global__ void k_sort_count(const int *__restrict__ key, const SortChunk *__restrict__ chunks,
+                             int nbins, int *__restrict__ H)
+{
+    extern __shared__ int hist[];
+    const SortChunk ch = chunks[blockIdx.x];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    for (int64_t p = ch.p0 + threadIdx.x; p < ch.p1; p += blockDim.x)
+        __hip_atomic_fetch_add(&hist[key[p]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) H[ch.hbase + (int64_t)b * ch.nch + ch.lc] = hist[b];
+}
+
+// exclusive scan, three phases over blocks of SCAN_B elements
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *sh, int64_t &total)
+{
+    // blockDim.x == 256, wave64
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wv] = x;
+    __syncthreads();
+    int64_t wbase = 0;
+    for (int k = 0; k < wv; ++k) wbase += sh[k];
+    total = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return wbase + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partial(const int *__restrict__ H, int64_t n, int per_thread,
+                                                      int64_t *__restrict__ bsum)
+{
+    __shared__ int64_t sh[4];
+    const int64_t base = ((int64_t)blockIdx.x * 256 + threadIdx.x) * per_thread;
+    int64_t s = 0;
+    for (int k = 0; k < per_thread; ++k)
+        if (base + k < n) s += H[base + k];
+    int64_t tot;
+    block_excl_scan(s, sh, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_scan_top(int64_t *__restrict__ bsum, int nb)
+{
+    __shared__ int64_t sh[4];
+    int64_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+        const int i = b0 + threadIdx.x;
+        const int64_t v = (i < nb) ? bsum[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_excl_scan(v, sh, tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_final(const int *__restrict__ H, int64_t n, int per_thread,
+                                                    const int64_t *__restrict__ bsum, int64_t *__restrict__ Hs)
+{
+    __shared__ int64_t sh[4];
+    const int64_t base = ((int64_t)blockIdx.x * 256 + threadIdx.x) * per_thread;
+    int64_t s = 0;
+    for (int k = 0; k < per_thread; ++k)
+        if (base + k < n) s += H[base + k];
+    int64_t tot;
+    int64_t run = bsum[blockIdx.x] + block_excl_scan(s, sh, tot);
+    for (int k = 0; k < per_thread; ++k)
+        if (base + k < n) {
+            Hs[base + k] = run;
+            run += H[base + k];
+        }
+}
+
+// Stable scatter.  Each block owns one chunk and walks it in rounds of
+// blockDim points; inside a round the waves take turns in order, and inside
+// a wave lanes with equal keys are ranked by lane id (bit-sliced ballots), so
+// the slot of a point is (#earlier points of the chunk with its key) +
+// (destination of its (bin, chunk) cell).
+template <typename R>
+__global__ __launch_bounds__(256) void k_sort_scatter(const int *__restrict__ key, const SortChunk *__restrict__ chunks,
+                                                      int nbins, int keybits, const int64_t *__restrict__ Hs,
+                                                      const PtRec<R> *__restrict__ rec,
+                                                      const typename cplx<R>::T *__restrict__ cin,
+                                                      PtRec<R> *__restrict__ srec,
+                                                      typename cplx<R>::T *__restrict__ cout,
+                                                      int *__restrict__ perm_out)
+{
+    extern __shared__ int cnt[];
+    const SortChunk ch = chunks[blockIdx.x];
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int64_t r0 = ch.p0; r0 < ch.p1; r0 += blockDim.x) {
+        const int64_t p = r0 + threadIdx.x;
+        const bool valid = p < ch.p1;
+        const int k = valid ? key[p] : 0;
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < keybits; ++b) {
+            const int bit = (k >> b) & 1;
+            const uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        const int rin = __popcll(m & below);
+        const int same = __popcll(m);
+        int rank = 0;
+        for (int w = 0; w < nw; ++w) {
+            if (wv == w && valid) {
+                const int base = cnt[k];
+                rank = base + rin;
+                if (rin == 0) cnt[k] = base + same;
+            }
+            __syncthreads();
+        }
+        if (valid) {
+            const int64_t dst = Hs[ch.hbase + (int64_t)k * ch.nch + ch.lc] + rank;
+            srec[dst] = rec[p];
+            if (cin) cout[dst] = cin[p];
+            if (perm_out) perm_out[dst] = (int)(p - ch.pbase);
+        }
+    }
+}
+
+// binstart[t][b] = first sorted slot of bin b of transform t; [t][nbins] = end
+__global__ void k_binstart(const int64_t *__restrict__ Hs, const TransformInfo *__restrict__ tinfo, int ntr,
+                           int nbins, int64_t *__restrict__ binstart)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ntr * (nbins + 1)) return;
+    const int t = (int)(i / (nbins + 1)), b = (int)(i % (nbins + 1));
+    const TransformInfo ti = tinfo[t];
+    binstart[i] = (b < nbins && ti.nch > 0) ? Hs[ti.hbase + (int64_t)b * ti.nch] : ti.p1;
+}
+
+// strengths of the plan API: cs[tr][k] = c[tr][perm[k]] (conjugated for iflag > 0)
+template <typename R>
+__global__ void k_gather_strengths(int64_t M, int ntr, const int *__restrict__ perm,
+                                   const typename cplx<R>::T *__restrict__ c, typename cplx<R>::T *__restrict__ cs,
+                                   int conj)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * ntr) return;
+    const int64_t tr = i / M, k = i % M;
+    auto v = c[tr * M + perm[k]];
+    if (conj) v.y = -v.y;
+    cs[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// spreading + x FFT
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+__device__ __forceinline__ int posmod(int a, int n)
+{
+    int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+template <typename R> __device__ __forceinline__ R es_kernel(R t, R beta, R ic)
+{
+    const R a = (R)1 - t * t * ic;
+    return (a > (R)0) ? exp(beta * (sqrt(a) - (R)1)) : (R)0;
+}
+
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const PtRec<R> *__restrict__ srec,
+                                                    const typename cplx<R>::T *__restrict__ cs, int64_t c_tstride,
+                                                    const int64_t *__restrict__ binstart, int bin_tstride,
+                                                    const typename cplx<R>::T *__restrict__ tw1,
+                                                    const R *__restrict__ phinv1, typename cplx<R>::T *__restrict__ G1,
+                                                    int final_conj)
+{
+    using C = typename cplx<R>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *tile = reinterpret_cast<C *>(smem_raw);
+    __shared__ int64_t rstart[MAX_NB_BINS];
+    __shared__ int rpref[MAX_NB_BINS + 1];
+    __shared__ int nrange;
+
+    const int t = blockIdx.y;
+    const int n1 = g.n1;
+    const int by = (g.dim >= 2) ? g.by : 1;
+    const int bz = (g.dim >= 3) ? g.bz : 1;
+    const int rows = by * bz;
+    const int nty = (g.dim >= 2) ? g.n2 / by : 1;
+    const int ty = blockIdx.x % nty, tz = blockIdx.x / nty;
+    const int y0 = ty * by, z0 = tz * bz;
+
+    for (int i = threadIdx.x; i < rows * n1; i += NT) tile[i] = cmk<C>((R)0, (R)0);
+
+    if (threadIdx.x == 0) {
+        const int64_t *bs = binstart + (int64_t)t * bin_tstride * (g.nbins + 1);
+        int ylo = 0, nyb = 1, zlo = 0, nzb = 1;
+        if (g.dim >= 2) {
+            ylo = floordiv(y0 - (g.w - 1), g.BY);
+            nyb = floordiv(y0 + by - 1, g.BY) - ylo + 1;
+            if (nyb >= g.nbY) { ylo = 0; nyb = g.nbY; }
+        }
+        if (g.dim >= 3) {
+            zlo = floordiv(z0 - (g.w - 1), g.BZ);
+            nzb = floordiv(z0 + bz - 1, g.BZ) - zlo + 1;
+            if (nzb >= g.nbZ) { zlo = 0; nzb = g.nbZ; }
+        }
+        int k = 0, acc = 0;
+        for (int iz = 0; iz < nzb; ++iz)
+            for (int iy = 0; iy < nyb; ++iy) {
+                const int b = (g.dim >= 3 ? posmod(zlo + iz, g.nbZ) * g.nbY : 0) + (g.dim >= 2 ? posmod(ylo + iy, g.nbY) : 0);
+                const int64_t s0 = bs[b], s1 = bs[b + 1];
+                if (s1 > s0 && k < MAX_NB_BINS) {
+                    rstart[k] = s0;
+                    rpref[k] = acc;
+                    acc += (int)(s1 - s0);
+                    ++k;
+                }
+            }
+        rpref[k] = acc;
+        nrange = k;
+    }
+    __syncthreads();
+
+    const int nr = nrange;
+    const int total = rpref[nr];
+    const R beta = (R)g.beta, ic = (R)g.ic;
+    const int w = g.w;
+    const C *cst = cs + (int64_t)t * c_tstride;
+    for (int v = threadIdx.x; v < total; v += NT) {
+        int k = 0;
+        while (k + 1 < nr && rpref[k + 1] <= v) ++k;
+        const int64_t p = rstart[k] + (v - rpref[k]);
+        const PtRec<R> pr = srec[p];
+        const C cv = cst[p];
+        // rows of the kernel inside the tile
+        R ky[MAXW], kz[MAXW];
+        int ry[MAXW], rz[MAXW];
+        int nyr = 0, nzr = 0;
+        if (g.dim >= 2) {
+            int oy = pr.ly - y0;
+            if (oy < 0) oy += g.n2;
+            for (int m = 0; m < w; ++m) {
+                int q = oy + m;
+                if (q >= g.n2) q -= g.n2;
+                if (q < by) {
+                    ry[nyr] = q;
+                    ky[nyr] = es_kernel<R>(pr.dy + (R)m, beta, ic);
+                    ++nyr;
+                }
+            }
+        } else {
+            ry[0] = 0;
+            ky[0] = (R)1;
+            nyr = 1;
+        }
+        if (g.dim >= 3) {
+            int oz = pr.lz - z0;
+            if (oz < 0) oz += g.n3;
+            for (int m = 0; m < w; ++m) {
+                int q = oz + m;
+                if (q >= g.n3) q -= g.n3;
+                if (q < bz) {
+                    rz[nzr] = q;
+                    kz[nzr] = es_kernel<R>(pr.dz + (R)m, beta, ic);
+                    ++nzr;
+                }
+            }
+        } else {
+            rz[0] = 0;
+            kz[0] = (R)1;
+            nzr = 1;
+        }
+        if (nyr == 0 || nzr == 0) continue;
+        R kx[MAXW];
+        for (int m = 0; m < w; ++m) kx[m] = es_kernel<R>(pr.dx + (R)m, beta, ic);
+        for (int iz = 0; iz < nzr; ++iz)
+            for (int iy = 0; iy < nyr; ++iy) {
+                const R wyz = ky[iy] * kz[iz];
+                const R vr = cv.x * wyz, vi = cv.y * wyz;
+                C *row = tile + (rz[iz] * by + ry[iy]) * n1;
+                int xi = pr.lx;
+                for (int m = 0; m < w; ++m) {
+                    R *cell = reinterpret_cast<R *>(row + xi);
+                    __hip_atomic_fetch_add(cell, vr * kx[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cell + 1, vi * kx[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (++xi == n1) xi = 0;
+                }
+            }
+    }
+    __syncthreads();
+
+    lds_fft<C, E>(tile, rp1, rows, 1, n1, tw1, false);
+
+    // keep k1 in [-N1/2, (N1-1)/2] (CMCL index kk = k1 + N1/2), deconvolve
+    const int N1 = g.N1;
+    const int h1 = N1 / 2;
+    for (int i = threadIdx.x; i < rows * N1; i += NT) {
+        const int row = i / N1, kk = i % N1;
+        int f = kk - h1;
+        if (f < 0) f += n1;
+        C v = cscale(tile[row * n1 + f], phinv1[kk]);
+        if (final_conj) v.y = -v.y;
+        const int yy = y0 + row % by, zz = z0 + row / by;
+        G1[(((int64_t)t * g.n3 + zz) * g.n2 + yy) * N1 + kk] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column FFT pass (y or z): transform t holds [outer][n][inner]; the block
+// transforms `cols` adjacent columns of one outer slab along n in LDS,
+// keeps N of the n frequencies, deconvolves.  FINAL: additionally sums the
+// nsp transforms of group blockIdx.y with weights and writes [g][N][inner]
+// (conjugated for iflag > 0).
+// ---------------------------------------------------------------------------
+template <typename R, int E, int NT, bool FINAL>
+__global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const typename cplx<R>::T *__restrict__ in,
+                                               typename cplx<R>::T *__restrict__ out,
+                                               const typename cplx<R>::T *__restrict__ tw,
+                                               const R *__restrict__ phinv, const R *__restrict__ wsc,
+                                               const float *__restrict__ wtab)
+{
+    using C = typename cplx<R>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *tile = reinterpret_cast<C *>(smem_raw);
+    const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
+    const int ncb = (inner + cols - 1) / cols;
+    const int cb = blockIdx.x % ncb, o = blockIdx.x / ncb;
+    const int col0 = cb * cols;
+    const int ncol = min(cols, inner - col0);
+    const int h = N / 2;
+    constexpr int NACC = (E + 1) / 2 + 1;
+    C acc[NACC];
+#pragma unroll
+    for (int i = 0; i < NACC; ++i) acc[i] = cmk<C>((R)0, (R)0);
+
+    const int nsp = FINAL ? cp.nsp : 1;
+    for (int s = 0; s < nsp; ++s) {
+        const int t = FINAL ? blockIdx.y * cp.nsp + s : blockIdx.y;
+        const C *src = in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
+        for (int i = threadIdx.x; i < n * cols; i += NT) {
+            const int r = i / cols, c = i % cols;
+            tile[i] = (c < ncol) ? src[(int64_t)r * inner + c] : cmk<C>((R)0, (R)0);
+        }
+        __syncthreads();
+        lds_fft<C, E>(tile, rp, cols, cols, 1, tw, true);
+        if (!FINAL) {
+            C *dst = out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
+            for (int i = threadIdx.x; i < N * cols; i += NT) {
+                const int kk = i / cols, c = i % cols;
+                if (c >= ncol) continue;
+                int f = kk - h;
+                if (f < 0) f += n;
+                dst[(int64_t)kk * inner + c] = cscale(tile[f * cols + c], phinv[kk]);
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < NACC; ++a) {
+                const int i = threadIdx.x + a * NT;
+                if (i < N * cols) {
+                    const int kk = i / cols, c = i % cols;
+                    if (c < ncol) {
+                        int f = kk - h;
+                        if (f < 0) f += n;
+                        R wv = phinv[kk];
+                        if (wtab) wv *= (R)wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c];
+                        else if (wsc) wv *= wsc[s];
+                        const C v = tile[f * cols + c];
+                        acc[a].x += v.x * wv;
+                        acc[a].y += v.y * wv;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (FINAL) {
+        C *dst = out + (int64_t)blockIdx.y * cp.out_tstride + (int64_t)o * N * inner + col0;
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) {
+            const int i = threadIdx.x + a * NT;
+            if (i < N * cols) {
+                const int kk = i / cols, c = i % cols;
+                if (c < ncol) {
+                    C v = acc[a];
+                    if (cp.conj) v.y = -v.y;
+                    dst[(int64_t)kk * inner + c] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations + launch helpers
+// ---------------------------------------------------------------------------
+template <typename R> struct Tiles;
+template <> struct Tiles<float> { static constexpr int E = 32, NT = 256; };
+template <> struct Tiles<double> { static constexpr int E = 16, NT = 256; };
+
+template <typename R>
+void launch_fold(const Geo &g, int64_t M, const void *x, const void *y, const void *z, bool in_f32, PtRec<R> *rec,
+                 int *key, int *bad, hipStream_t st)
+{
+    const int NT = 256;
+    const unsigned nb = (unsigned)((M + NT - 1) / NT);
+    if (M == 0) return;
+    if (in_f32)
+        hipLaunchKernelGGL((k_fold<R, float>), dim3(nb), dim3(NT), 0, st, g, M, (const float *)x, (const float *)y,
+                           (const float *)z, rec, key, bad);
+    else
+        hipLaunchKernelGGL((k_fold<R, double>), dim3(nb), dim3(NT), 0, st, g, M, (const double *)x,
+                           (const double *)y, (const double *)z, rec, key, bad);
+}
+
+template <typename R>
+void launch_stage(const Geo &g, const StageParams &sp, int64_t npts, const float4 *pos, const int4 *at_ind,
+                  PtRec<R> *rec, int *key, typename cplx<R>::T *cstr, int *bad, hipStream_t st)
+{
+    const int NT = 256;
+    if (npts == 0) return;
+    hipLaunchKernelGGL((k_stage_traj<R>), dim3((unsigned)((npts + NT - 1) / NT)), dim3(NT), 0, st, g, sp, npts, pos,
+                       at_ind, rec, key, cstr, bad);
+}
+
+void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int nbins, int *H, hipStream_t st)
+{
+    if (nchunks == 0) return;
+    hipLaunchKernelGGL(k_sort_count, dim3(nchunks), dim3(256), nbins * sizeof(int), st, key, chunks, nbins, H);
+}
+
+void launch_scan(const int *H, int64_t n, int64_t *bsum, int64_t *Hs, hipStream_t st)
+{
+    if (n == 0) return;
+    const int per = 16;
+    const int64_t per_block = 256LL * per;
+    const int nb = (int)((n + per_block - 1) / per_block);
+    hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, st, H, n, per, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, st, H, n, per, bsum, Hs);
+}
+
+int64_t scan_bsum_size(int64_t n) { return (n + 256LL * 16 - 1) / (256LL * 16) + 1; }
+
+template <typename R>
+void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int nbins, int keybits,
+                         const int64_t *Hs, const PtRec<R> *rec, const typename cplx<R>::T *cin, PtRec<R> *srec,
+                         typename cplx<R>::T *cout, int *perm, hipStream_t st)
+{
+    if (nchunks == 0) return;
+    hipLaunchKernelGGL((k_sort_scatter<R>), dim3(nchunks), dim3(256), nbins * sizeof(int), st, key, chunks, nbins,
+                       keybits, Hs, rec, cin, srec, cout, perm);
+}
+
+void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int nbins, int64_t *binstart,
+                     hipStream_t st)
+{
+    const int64_t n = (int64_t)ntr * (nbins + 1);
+    hipLaunchKernelGGL(k_binstart, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Hs, tinfo, ntr, nbins,
+                       binstart);
+}
+
+template <typename R>
+void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::T *c, typename cplx<R>::T *cs,
+                   int conj, hipStream_t st)
+{
+    const int64_t n = M * ntr;
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_gather_strengths<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, M, ntr, perm, c,
+                       cs, conj);
+}
+
+template <typename R> int spread_tile_elems() { return Tiles<R>::E * Tiles<R>::NT; }
+
+template <typename R>
+void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
+                        const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
+                        const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
+                        hipStream_t st)
+{
+    constexpr int E = Tiles<R>::E, NT = Tiles<R>::NT;
+    const int by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
+    const int ntiles = ((g.dim >= 2) ? g.n2 / by : 1) * ((g.dim >= 3) ? g.n3 / bz : 1);
+    const size_t lds = (size_t)by * bz * g.n1 * sizeof(typename cplx<R>::T);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_spread_xfft<R, E, NT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_spread_xfft<R, E, NT>), dim3(ntiles, ntr), dim3(NT), lds, st, g, rp1, srec, cs, c_tstride,
+                       binstart, bin_tstride, tw1, phinv1, G1, final_conj);
+}
+
+template <typename R>
+void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
+                   typename cplx<R>::T *out, const typename cplx<R>::T *tw, const R *phinv, const R *wsc,
+                   const float *wtab, bool final, hipStream_t st)
+{
+    constexpr int E = Tiles<R>::E, NT = Tiles<R>::NT;
+    const int ncb = (cp.inner + cp.cols - 1) / cp.cols;
+    const size_t lds = (size_t)cp.n * cp.cols * sizeof(typename cplx<R>::T);
+    dim3 grid(ncb * cp.outer, ngroups);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NT, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NT, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        attr = true;
+    }
+    if (final)
+        hipLaunchKernelGGL((k_colfft<R, E, NT, true>), grid, dim3(NT), lds, st, cp, rp, in, out, tw, phinv, wsc, wtab);
+    else
+        hipLaunchKernelGGL((k_colfft<R, E, NT, false>), grid, dim3(NT), lds, st, cp, rp, in, out, tw, phinv, wsc,
+                           wtab);
+}
+
+#define MPSQ_INST(R)                                                                                                  \
+    template void launch_fold<R>(const Geo &, int64_t, const void *, const void *, const void *, bool, PtRec<R> *,    \
+                                 int *, int *, hipStream_t);                                                          \
+    template void launch_stage<R>(const Geo &, const StageParams &, int64_t, const float4 *, const int4 *,           \
+                                  PtRec<R> *, int *, cplx<R>::T *, int *, hipStream_t);                               \
+    template void launch_sort_scatter<R>(const int *, const SortChunk *, int, int, int, const int64_t *,             \
+                                         const PtRec<R> *, const cplx<R>::T *, PtRec<R> *, cplx<R>::T *, int *,       \
+                                         hipStream_t);                                                                \
+    template void launch_gather<R>(int64_t, int, const int *, const cplx<R>::T *, cplx<R>::T *, int, hipStream_t);    \
+    template int spread_tile_elems<R>();                                                                              \
+    template void launch_spread_xfft<R>(const Geo &, const RadixPlan &, int, const PtRec<R> *, const cplx<R>::T *,   \
+                                        int64_t, const int64_t *, int, const cplx<R>::T *, const R *, cplx<R>::T *,   \
+                                        int, hipStream_t);                                                            \
+    template void launch_colfft<R>(const ColPass &, const RadixPlan &, int, const cplx<R>::T *, cplx<R>::T *,        \
+                                   const cplx<R>::T *, const R *, const R *, const float *, bool, hipStream_t);
+
+MPSQ_INST(float)
+MPSQ_INST(double)
+
+}  // namespace mpsq

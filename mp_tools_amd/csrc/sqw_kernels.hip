@@ -1,0 +1,233 @@
+// sqw_kernels.hip -- S(Q) / S(Q,w) reduction of combined amplitudes C(t,Q)
+// (layout [T][NQ], Q fastest) on gfx950.
+//
+// k_sqw_single   one Hann window over all T frames (n_int == T): a block
+//                stages T x cols amplitudes in LDS, forms the per-Q mean and
+//                sum |C|^2 in FP64, subtracts the mean, applies the window
+//                (and the optional per-Q factor), runs the time FFT down the
+//                LDS columns, and writes |X(m)|^2 in FP64
+//                (mp_sqom56.f90:1177-1185, 1381-1387, 1480-1517, 1527, 1576).
+// k_sqw_sliding  sliding windows with hop 1 (n_frame = T - n_int + 1 > 1),
+//                any n_int: the Hann window is a sum of three exponentials,
+//                so each windowed DFT X_s(m) is a combination of three
+//                rectangular sliding DFTs that advance by an O(1) recurrence
+//                per frame (FP64); sum_s |X_s(m)|^2 accumulates in FP64
+//                (mp_sqom56.f90:1437-1478 computes the same sum per energy).
+#include "mpsq_kernels.h"
+
+namespace mpsq {
+
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, int cols,
+                                                   const typename cplx<R>::T *__restrict__ amp,
+                                                   const typename cplx<R>::T *__restrict__ tw,
+                                                   const double *__restrict__ win, const float *__restrict__ qfac,
+                                                   double *__restrict__ S, double *__restrict__ Stot)
+{
+    using C = typename cplx<R>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *tile = reinterpret_cast<C *>(smem_raw);
+    __shared__ double red[3 * NT];
+    __shared__ double mean_r[64], mean_i[64];
+    const int T = (int)p.T;
+    const int64_t q0 = (int64_t)blockIdx.x * cols;
+    const int ncol = (int)min((int64_t)cols, p.NQ - q0);
+    for (int i = threadIdx.x; i < T * cols; i += NT) {
+        const int t = i / cols, c = i % cols;
+        tile[i] = (c < ncol) ? amp[(int64_t)t * p.NQ + q0 + c] : cmk<C>((R)0, (R)0);
+    }
+    __syncthreads();
+    {
+        const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
+        double sr = 0.0, si = 0.0, ss = 0.0;
+        if (seg < nseg)
+            for (int t = seg; t < T; t += nseg) {
+                const C v = tile[t * cols + c];
+                sr += (double)v.x;
+                si += (double)v.y;
+                ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+            }
+        red[threadIdx.x] = sr;
+        red[NT + threadIdx.x] = si;
+        red[2 * NT + threadIdx.x] = ss;
+        __syncthreads();
+        if (threadIdx.x < cols) {
+            double a = 0.0, b = 0.0, q = 0.0;
+            for (int k = 0; k < nseg; ++k) {
+                a += red[k * cols + threadIdx.x];
+                b += red[NT + k * cols + threadIdx.x];
+                q += red[2 * NT + k * cols + threadIdx.x];
+            }
+            mean_r[threadIdx.x] = p.subtract_mean ? a / (double)T : 0.0;
+            mean_i[threadIdx.x] = p.subtract_mean ? b / (double)T : 0.0;
+            if (Stot && threadIdx.x < ncol) Stot[q0 + threadIdx.x] = q * p.norm_tot;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < T * cols; i += NT) {
+        const int t = i / cols, c = i % cols;
+        const C v = tile[i];
+        double f = win[t];
+        if (qfac && c < ncol) f *= (double)qfac[q0 + c];
+        tile[i] = cmk<C>((R)(((double)v.x - mean_r[c]) * f), (R)(((double)v.y - mean_i[c]) * f));
+    }
+    __syncthreads();
+    lds_fft<C, E>(tile, rp, cols, cols, 1, tw, true);
+    for (int i = threadIdx.x; i < p.n_f * cols; i += NT) {
+        const int mi = i / cols, c = i % cols;
+        if (c >= ncol) continue;
+        int m = (int)((p.m_lo + mi) % T);
+        if (m < 0) m += T;
+        const C v = tile[m * cols + c];
+        S[(int64_t)mi * p.NQ + q0 + c] = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * p.norm_s;
+    }
+}
+
+// sliding windows: thread = (Q, group of MG frequencies)
+template <typename R, int MG>
+__global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
+                                                     const float *__restrict__ qfac, double *__restrict__ S,
+                                                     double *__restrict__ Stot)
+{
+    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int grp = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (q >= p.NQ || grp * MG >= p.n_f) return;
+    const int64_t T = p.T;
+    const int n = p.n_int;
+    double mr = 0.0, mi = 0.0, ss = 0.0;
+    for (int64_t t = 0; t < T; ++t) {
+        const auto v = amp[t * p.NQ + q];
+        mr += (double)v.x;
+        mi += (double)v.y;
+        ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    if (grp == 0 && Stot) Stot[q] = ss * p.norm_tot;
+    if (p.subtract_mean) {
+        mr /= (double)T;
+        mi /= (double)T;
+    } else {
+        mr = mi = 0.0;
+    }
+    const double qf = qfac ? (double)qfac[q] : 1.0;
+    const double twopi = 6.283185307179586476925286766559;
+    // window = a0 + a1 e^{i th i} + a2 e^{-i th i}
+    double th = 0.0, a0 = 1.0, a1r = 0.0, a1i = 0.0;
+    if (n > 1) {
+        const int off = p.win_kind == 1 ? 1 : 0;
+        th = twopi / (double)(p.win_kind == 1 ? n + 1 : n - 1);
+        a0 = 0.5;
+        a1r = -0.25 * cos(th * off);
+        a1i = -0.25 * sin(th * off);
+    }
+    // a2 = conj(a1)
+    double Dr[MG][3], Di[MG][3], rr[MG][3], ri[MG][3], trr[MG][3], tri[MG][3], sr[MG][3], si[MG][3];
+    double acc[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) {
+        int64_t m = (p.m_lo + (int64_t)(grp * MG + j)) % n;
+        if (m < 0) m += n;
+        const double om = twopi * (double)m / (double)n;
+        const double oms[3] = {om, om - th, om + th};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Dr[j][k] = Di[j][k] = 0.0;
+            rr[j][k] = cos(oms[k]);          // e^{+i w}
+            ri[j][k] = sin(oms[k]);
+            trr[j][k] = cos(oms[k] * (double)(n - 1));   // e^{-i w (n-1)}
+            tri[j][k] = -sin(oms[k] * (double)(n - 1));
+            sr[j][k] = 1.0;                  // running e^{-i w i}
+            si[j][k] = 0.0;
+        }
+        acc[j] = 0.0;
+    }
+    for (int i = 0; i < n; ++i) {
+        const auto v = amp[(int64_t)i * p.NQ + q];
+        const double cr = ((double)v.x - mr) * qf, ci = ((double)v.y - mi) * qf;
+#pragma unroll
+        for (int j = 0; j < MG; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Dr[j][k] += sr[j][k] * cr - si[j][k] * ci;
+                Di[j][k] += sr[j][k] * ci + si[j][k] * cr;
+                // multiply running twiddle by e^{-i w}
+                const double nr = sr[j][k] * rr[j][k] + si[j][k] * ri[j][k];
+                const double ni = si[j][k] * rr[j][k] - sr[j][k] * ri[j][k];
+                sr[j][k] = nr;
+                si[j][k] = ni;
+            }
+    }
+    for (int64_t s = 0;; ++s) {
+#pragma unroll
+        for (int j = 0; j < MG; ++j) {
+            // X = a0 D0 + a1 D1 + conj(a1) D2
+            const double xr = a0 * Dr[j][0] + (a1r * Dr[j][1] - a1i * Di[j][1]) + (a1r * Dr[j][2] + a1i * Di[j][2]);
+            const double xi = a0 * Di[j][0] + (a1r * Di[j][1] + a1i * Dr[j][1]) + (a1r * Di[j][2] - a1i * Dr[j][2]);
+            acc[j] += xr * xr + xi * xi;
+        }
+        if (s + 1 >= p.n_frame) break;
+        const auto vo = amp[s * p.NQ + q];
+        const auto vn = amp[(s + n) * p.NQ + q];
+        const double or_ = ((double)vo.x - mr) * qf, oi = ((double)vo.y - mi) * qf;
+        const double nr_ = ((double)vn.x - mr) * qf, ni = ((double)vn.y - mi) * qf;
+#pragma unroll
+        for (int j = 0; j < MG; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double ar = Dr[j][k] - or_, ai = Di[j][k] - oi;
+                Dr[j][k] = (ar * rr[j][k] - ai * ri[j][k]) + (trr[j][k] * nr_ - tri[j][k] * ni);
+                Di[j][k] = (ar * ri[j][k] + ai * rr[j][k]) + (trr[j][k] * ni + tri[j][k] * nr_);
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < MG; ++j) {
+        const int mi_ = grp * MG + j;
+        if (mi_ < p.n_f) S[(int64_t)mi_ * p.NQ + q] = acc[j] * p.norm_s;
+    }
+}
+
+template <typename R> struct SqwTiles;
+template <> struct SqwTiles<float> { static constexpr int E = 32, NT = 256; };
+template <> struct SqwTiles<double> { static constexpr int E = 16, NT = 256; };
+
+template <typename R> int sqw_tile_elems() { return SqwTiles<R>::E * SqwTiles<R>::NT; }
+
+template <typename R>
+void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,
+                       const typename cplx<R>::T *tw, const double *win, const float *qfac, double *S, double *Stot,
+                       hipStream_t st)
+{
+    constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
+    const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
+    const size_t lds = (size_t)p.T * cols * sizeof(typename cplx<R>::T);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqw_single<R, E, NT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_sqw_single<R, E, NT>), dim3(nb), dim3(NT), lds, st, p, rp, cols, amp, tw, win, qfac, S,
+                       Stot);
+}
+
+template <typename R>
+void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S,
+                        double *Stot, hipStream_t st)
+{
+    constexpr int MG = 4;
+    const unsigned gx = (unsigned)((p.NQ + 63) / 64);
+    const unsigned gy = (unsigned)((p.n_f + 4 * MG - 1) / (4 * MG));
+    hipLaunchKernelGGL((k_sqw_sliding<R, MG>), dim3(gx, gy > 0 ? gy : 1), dim3(256), 0, st, p, amp, qfac, S, Stot);
+}
+
+#define MPSQ_SQW_INST(R)                                                                                              \
+    template int sqw_tile_elems<R>();                                                                                 \
+    template void launch_sqw_single<R>(const SqwParams &, const RadixPlan &, int, const cplx<R>::T *,                \
+                                       const cplx<R>::T *, const double *, const float *, double *, double *,        \
+                                       hipStream_t);                                                                  \
+    template void launch_sqw_sliding<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *,    \
+                                        hipStream_t);
+
+MPSQ_SQW_INST(float)
+MPSQ_SQW_INST(double)
+
+}  // namespace mpsq

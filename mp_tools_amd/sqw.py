@@ -1,0 +1,163 @@
+"""S(Q) and S(Q,w) maps from device-resident trajectories.
+
+Python face of the ``mpsq_sqw_*`` C ABI (``include/mpsq.h``).  The reference
+computes, per frame and species, a type-1 NUFFT of staged positions
+(code/src/mp_sqom56.f90:1101-1156), combines the species with weights
+(:1338-1377), and reduces the amplitudes to the total map (:1381-1387) or to
+S(Q,w) by a Hann-windowed time transform (:1437-1517) with the normalisation
+of :1527 and :1572-1577.  ``SqwEngine.amplitudes`` runs the first two steps,
+``SqwEngine.reduce`` the last, both on the GPU.
+
+Device memory and streams come from torch (plumbing only); all arithmetic
+runs in libmpsq's HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class SqwEngine:
+    """Per-species NUFFT + species combine + time reduction on one GPU.
+
+    Parameters mirror the reference's inputs: ``species_counts`` are the
+    nsuper_r of each species in species-major storage order, ``weights``
+    the per-species scalars (1 or 0.1*b_coh), ``weight_table`` per-Q weights
+    [nsp, NQ] (X-ray form factors), ``mask`` the at_mask species set.
+    Geometry: ``axes`` rows are the unit plane vectors, ``tpe`` the phase
+    scale 2 pi / (e_d . (pmax - pmin)), ``centre`` the real-space centre,
+    ``q0`` the Q-centre of the strengths' phase exp(-i 2 pi q0 . r).
+    """
+
+    def __init__(self, dim, n_modes, species_counts, eps=1e-6, precision="fp32", iflag=-1,
+                 weights=None, weight_table=None, mask=None, combine=True,
+                 centre=(0.0, 0.0, 0.0), axes=None, tpe=(1.0, 1.0, 1.0), q0=(0.0, 0.0, 0.0),
+                 window=False, vacancy_check=True, select_mode=0, n_row=(1, 1, 1),
+                 pmin=(0.0, 0.0, 0.0), pmax=(0.0, 0.0, 0.0), margin=0.1, at_ind=None,
+                 batch_frames=0, device=0, stream="torch", tile_by=0, tile_bz=0):
+        import torch
+
+        self._L = _lib.load()
+        self.dim = int(dim)
+        self.n_modes = tuple(int(v) for v in n_modes)
+        if len(self.n_modes) != self.dim:
+            raise ValueError("n_modes must have dim entries")
+        self.NQ = int(np.prod(self.n_modes))
+        self.precision = precision
+        self.fp64 = precision == "fp64"
+        self.cdtype = torch.complex128 if self.fp64 else torch.complex64
+        counts = np.ascontiguousarray(species_counts, np.int64)
+        self.nsp = counts.size
+        self.n_tot = int(counts.sum())
+        s = _lib.SqwSpec()
+        self._L.mpsq_sqw_default_spec(C.byref(s))
+        s.dim = self.dim
+        for d in range(3):
+            s.n_modes[d] = self.n_modes[d] if d < self.dim else 1
+        s.eps = float(eps)
+        s.precision = _lib.FP64 if self.fp64 else _lib.FP32
+        s.iflag = int(iflag)
+        s.n_tot = self.n_tot
+        s.nsp = self.nsp
+        self._keep = [counts]
+        s.sp_count = counts.ctypes.data
+        if mask is not None:
+            m = np.ascontiguousarray(mask, np.int32)
+            self._keep.append(m)
+            s.mask = m.ctypes.data
+        if weights is not None:
+            w = np.ascontiguousarray(weights, np.float64)
+            self._keep.append(w)
+            s.w_scalar = w.ctypes.data
+        if weight_table is not None:
+            wt = np.ascontiguousarray(weight_table, np.float32).reshape(self.nsp, self.NQ)
+            self._keep.append(wt)
+            s.w_table = wt.ctypes.data
+        s.combine = 1 if combine else 0
+        self.combine = bool(combine)
+        ax = np.eye(3) if axes is None else np.asarray(axes, np.float64).reshape(3, 3)
+        for d in range(3):
+            s.centre[d] = float(centre[d])
+            s.tpe[d] = float(tpe[d])
+            s.q0[d] = float(q0[d])
+            s.n_row[d] = int(n_row[d])
+            s.pmin[d] = float(pmin[d])
+            s.pmax[d] = float(pmax[d])
+        for k in range(9):
+            s.axes[k] = float(ax.ravel()[k])
+        s.window = int(bool(window))
+        s.vacancy_check = int(bool(vacancy_check))
+        s.select_mode = int(select_mode)
+        s.margin = float(margin)
+        if at_ind is not None:
+            ai = np.ascontiguousarray(at_ind, np.int32).reshape(self.n_tot, 4)
+            self._keep.append(ai)
+            s.at_ind = ai.ctypes.data
+        s.batch_frames = int(batch_frames)
+        s.opts.device = int(device)
+        if stream == "torch":
+            torch.cuda.set_device(int(device))
+            stream = torch.cuda.current_stream().cuda_stream
+        s.opts.stream = stream
+        s.opts.tile_by, s.opts.tile_bz = int(tile_by), int(tile_bz)
+        h = C.c_void_p()
+        rc = self._L.mpsq_sqw_create(C.byref(s), C.byref(h))
+        _lib.check("sqw_create", rc)
+        self.warning = rc
+        self._h = h
+        self.nact = int(self._L.mpsq_sqw_nact(h))
+
+    def amplitudes(self, pos, out=None):
+        """pos: CUDA float32 tensor [T, n_tot, 4] (x, y, z, charge).
+        Returns C(t,Q) [T, NQ] (combine) or F_j(t,Q) [T, nact, NQ]."""
+        import torch
+
+        if pos.dtype != torch.float32 or not pos.is_cuda:
+            raise ValueError("pos must be a CUDA float32 tensor [T, n_tot, 4]")
+        pos = pos.contiguous()
+        T = pos.shape[0]
+        if pos.numel() != T * self.n_tot * 4:
+            raise ValueError("pos must have shape [T, n_tot, 4]")
+        shape = (T, self.NQ) if self.combine else (T, self.nact, self.NQ)
+        if out is None:
+            out = torch.empty(shape, dtype=self.cdtype, device=pos.device)
+        rc = self._L.mpsq_sqw_amplitudes(self._h, pos.data_ptr(), T, out.data_ptr())
+        _lib.check("sqw_amplitudes", rc)
+        return out
+
+    def reduce(self, amp, n_int=None, win_kind=0, dt=1.0, nat=1.0, m_lo=0, n_f=None,
+               subtract_mean=True, qfac=None, out=None, out_tot=None):
+        """S(Q,w) [n_f, NQ] and S_tot(Q) [NQ] (float64 CUDA tensors)."""
+        import torch
+
+        T = amp.shape[0]
+        n_int = T if n_int is None else int(n_int)
+        n_f = n_int if n_f is None else int(n_f)
+        r = _lib.ReduceSpec()
+        r.n_int, r.win_kind, r.dt, r.nat = n_int, int(win_kind), float(dt), float(nat)
+        r.m_lo, r.n_f, r.subtract_mean = int(m_lo), n_f, int(bool(subtract_mean))
+        if qfac is not None:
+            qf = qfac.to(torch.float32).contiguous()
+            r.d_qfac = qf.data_ptr()
+        if out is None:
+            out = torch.empty((n_f, self.NQ), dtype=torch.float64, device=amp.device)
+        if out_tot is None:
+            out_tot = torch.empty(self.NQ, dtype=torch.float64, device=amp.device)
+        rc = self._L.mpsq_sqw_reduce(self._h, amp.data_ptr(), T, C.byref(r), out.data_ptr(),
+                                     out_tot.data_ptr())
+        _lib.check("sqw_reduce", rc)
+        return out, out_tot
+
+    def destroy(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.mpsq_sqw_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

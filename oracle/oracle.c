@@ -1,0 +1,561 @@
+/*
+ * oracle.c -- CPU FP64 restatement of MP_tools' S(Q,w) hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+ * mp_tools_amd/: it may be built and called from tests/, from
+ * __graft_entry__.smoke() and from the cpu_baseline leg of bench.py, and from
+ * nowhere else.  The product path never links or calls it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the jkulda/MP_tools tree).  The non-uniform FFT itself
+ * lives in FINUFFT, which is not part of the reference tree; the oracle
+ * therefore restates the *definition* that the reference's own brute-force
+ * transform `dirft2d1` computes (code/src/mp_sqom56.f90:2530-2602) and the
+ * FINUFFT method as published (code/contrib/barnett_FINUFFT.txt:38-40).
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   orc_dirft_type1         uniform points reduce to numpy.fft; a single point
+ *                           at the origin; the shift theorem; a perfect
+ *                           lattice gives the closed-form Dirichlet pattern;
+ *                           linearity; 3D == stack of 2D slices with phase.
+ *   orc_dirft_type1_sampled equals orc_dirft_type1 at the sampled modes and
+ *                           a numpy brute force on tiny inputs.
+ *   orc_fold / orc_counting_sort  closed-form fold checked in numpy (floor /
+ *                           ceil / modulo) and numpy's stable argsort.
+ *   orc_stage               numpy restatement of the dot products + the
+ *                           single-point phase closed form.
+ *   orc_select_cell/bulk    known answers on the Al/U frame fixture and on
+ *                           hand-built vacancy/window cases.
+ *   orc_sqw_direct          T=1 closed form, constant amplitudes -> S==0,
+ *                           pure-tone closed form (peak at m0), numpy FFT of
+ *                           the windowed series, Parseval.
+ *   orc_es_spread           end-to-end NUFFT against orc_dirft_type1 at eps.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_TWO_PI (2.0 * M_PI)
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4 / a5: type-1 direct DFT.                                               */
+/* F(k1,k2,k3) = sum_j c_j exp(s*i*(k1 x_j + k2 y_j + k3 z_j)), s = +1 if     */
+/* iflag >= 0 else -1, k_d in [-N_d/2, (N_d-1)/2], output in CMCL order      */
+/* (k1 fastest).  Same definition as dirft2d1 (mp_sqom56.f90:2539-2546,       */
+/* sign rule :2549-2550, index range :2535); the 3D form is the stack of 2D   */
+/* slices with strengths c*exp(-i k3 z) that mp_sql builds                   */
+/* (mp_sql56.f90:1110-1132).  Like dirft2d1 it walks k1 with a per-point      */
+/* twiddle recurrence (mp_sqom56.f90:2564-2571); the (k2,k3) row factor is    */
+/* evaluated directly for accuracy.                                          */
+/* c and f are interleaved complex (re, im).                                 */
+/* ------------------------------------------------------------------------ */
+int orc_dirft_type1(int dim, int64_t M, const double *x, const double *y,
+                    const double *z, const double *c, int iflag,
+                    int64_t N1, int64_t N2, int64_t N3, double *f)
+{
+    if (dim < 1 || dim > 3 || M < 0 || N1 < 1 || N2 < 1 || N3 < 1) return 2;
+    if (dim < 2) N2 = 1;
+    if (dim < 3) N3 = 1;
+    const double s = (iflag >= 0) ? 1.0 : -1.0;
+    const int64_t nrow = N2 * N3;
+    const int64_t k1min = -(N1 / 2);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t r = 0; r < nrow; ++r) {
+        const int64_t k2 = (r % N2) - N2 / 2;
+        const int64_t k3 = (r / N2) - N3 / 2;
+        double *row = f + 2 * r * N1;
+        for (int64_t i = 0; i < 2 * N1; ++i) row[i] = 0.0;
+        for (int64_t j = 0; j < M; ++j) {
+            double ph = 0.0;
+            if (dim >= 2) ph += (double)k2 * y[j];
+            if (dim >= 3) ph += (double)k3 * z[j];
+            ph += (double)k1min * x[j];
+            const double er = cos(s * ph), ei = sin(s * ph);
+            double cr = c[2 * j] * er - c[2 * j + 1] * ei;
+            double ci = c[2 * j] * ei + c[2 * j + 1] * er;
+            const double zr = cos(s * x[j]), zi = sin(s * x[j]);
+            for (int64_t k = 0; k < N1; ++k) {
+                row[2 * k] += cr;
+                row[2 * k + 1] += ci;
+                const double tr = cr * zr - ci * zi;
+                ci = cr * zi + ci * zr;
+                cr = tr;
+            }
+        }
+    }
+    return 0;
+}
+
+/* Direct sum at K explicit integer modes (k1[m], k2[m], k3[m]).  Used at the  */
+/* full sizes where the dense direct DFT would take too long.  Points are     */
+/* processed in blocks; per block the 1D exponentials exp(s i k x) are        */
+/* tabulated with a recurrence re-anchored by an exact sincos every 16 steps, */
+/* so each (point, mode) term costs two complex products.                     */
+int orc_dirft_type1_sampled(int dim, int64_t M, const double *x,
+                            const double *y, const double *z, const double *c,
+                            int iflag, int64_t K, const int64_t *k1,
+                            const int64_t *k2, const int64_t *k3, double *f)
+{
+    if (dim < 1 || dim > 3 || M < 0 || K < 0) return 2;
+    const double s = (iflag >= 0) ? 1.0 : -1.0;
+    int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int64_t m = 0; m < K; ++m) {
+        const int64_t kk[3] = {k1[m], dim >= 2 ? k2[m] : 0, dim >= 3 ? k3[m] : 0};
+        for (int d = 0; d < 3; ++d) {
+            if (m == 0 || kk[d] < lo[d]) lo[d] = kk[d];
+            if (m == 0 || kk[d] > hi[d]) hi[d] = kk[d];
+        }
+    }
+    for (int64_t i = 0; i < 2 * K; ++i) f[i] = 0.0;
+    if (K == 0 || M == 0) return 0;
+    const int64_t span[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    const int64_t B = 256;
+    const int64_t nblk = (M + B - 1) / B;
+#pragma omp parallel
+    {
+        double *acc = (double *)calloc((size_t)(2 * K), sizeof(double));
+        double *tab[3];
+        for (int d = 0; d < 3; ++d)
+            tab[d] = (double *)malloc((size_t)(2 * span[d] * B) * sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t b = 0; b < nblk; ++b) {
+            const int64_t j0 = b * B;
+            const int64_t nb = (j0 + B <= M) ? B : (M - j0);
+            const double *co[3] = {x, y, z};
+            for (int d = 0; d < 3; ++d) {
+                for (int64_t jj = 0; jj < nb; ++jj) {
+                    const double xv = (d < dim) ? co[d][j0 + jj] : 0.0;
+                    const double zr = cos(s * xv), zi = sin(s * xv);
+                    double er = 0.0, ei = 0.0;
+                    for (int64_t q = 0; q < span[d]; ++q) {
+                        if ((q & 15) == 0) {
+                            const double ph = s * (double)(lo[d] + q) * xv;
+                            er = cos(ph);
+                            ei = sin(ph);
+                        }
+                        tab[d][2 * (q * B + jj)] = er;
+                        tab[d][2 * (q * B + jj) + 1] = ei;
+                        const double tr = er * zr - ei * zi;
+                        ei = er * zi + ei * zr;
+                        er = tr;
+                    }
+                }
+            }
+            for (int64_t m = 0; m < K; ++m) {
+                const double *t1 = tab[0] + 2 * (k1[m] - lo[0]) * B;
+                const double *t2 = tab[1] + 2 * ((dim >= 2 ? k2[m] : 0) - lo[1]) * B;
+                const double *t3 = tab[2] + 2 * ((dim >= 3 ? k3[m] : 0) - lo[2]) * B;
+                double sr = 0.0, si = 0.0;
+                for (int64_t jj = 0; jj < nb; ++jj) {
+                    const double ar = t1[2 * jj] * t2[2 * jj] - t1[2 * jj + 1] * t2[2 * jj + 1];
+                    const double ai = t1[2 * jj] * t2[2 * jj + 1] + t1[2 * jj + 1] * t2[2 * jj];
+                    const double br = ar * t3[2 * jj] - ai * t3[2 * jj + 1];
+                    const double bi = ar * t3[2 * jj + 1] + ai * t3[2 * jj];
+                    const double cr = c[2 * (j0 + jj)], ci = c[2 * (j0 + jj) + 1];
+                    sr += cr * br - ci * bi;
+                    si += cr * bi + ci * br;
+                }
+                acc[2 * m] += sr;
+                acc[2 * m + 1] += si;
+            }
+        }
+#pragma omp critical
+        for (int64_t i = 0; i < 2 * K; ++i) f[i] += acc[i];
+        free(acc);
+        for (int d = 0; d < 3; ++d) free(tab[d]);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4.1: FP64 periodic fold and bin key (the build's own definition; the     */
+/* FINUFFT internals it replaces are not in the reference tree).             */
+/* For each used dimension d: u = x * (n_d / 2pi); the kernel's first cell is */
+/* l = ceil(u - w/2); its offset d = l - u lies in [-w/2, -w/2+1); l is then  */
+/* wrapped into [0, n_d).  Points outside [-pi, pi) fold periodically.        */
+/* Bin of a point: (l_y / by, l_z / bz), key = (l_z/bz) * (n2/by) + l_y/by;   */
+/* in 1D every point is in bin 0.  The operation order below is fixed (no     */
+/* fused multiply-add; built with -ffp-contract=off) so that the GPU fold     */
+/* reproduces l, d and key bit for bit.                                       */
+/* ------------------------------------------------------------------------ */
+int orc_fold(int dim, int64_t M, const double *x, const double *y,
+             const double *z, const int64_t *n, int w, int by, int bz,
+             int32_t *l_out, double *d_out, int32_t *key_out)
+{
+    if (dim < 1 || dim > 3 || w < 2 || by < 1 || bz < 1) return 2;
+    const double *co[3] = {x, y, z};
+    const double hw = 0.5 * (double)w;
+    double sc[3];
+    for (int d = 0; d < 3; ++d) sc[d] = (double)n[d] / ORC_TWO_PI;
+    const int64_t nby = (dim >= 2) ? n[1] / by : 1;
+    int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+    for (int64_t j = 0; j < M; ++j) {
+        int32_t li[3] = {0, 0, 0};
+        for (int d = 0; d < 3; ++d) {
+            double dd = 0.0;
+            if (d < dim) {
+                const double xv = co[d][j];
+                if (!isfinite(xv)) { bad = 1; }
+                const double u = xv * sc[d];
+                const double t = u - hw;
+                const double l = ceil(t);
+                dd = l - u;
+                int64_t lw = (isfinite(l) && fabs(l) < 9.0e18) ? (int64_t)l : 0;
+                lw %= n[d];
+                if (lw < 0) lw += n[d];
+                li[d] = (int32_t)lw;
+            }
+            l_out[3 * j + d] = li[d];
+            d_out[3 * j + d] = dd;
+        }
+        int32_t key = 0;
+        if (dim == 2) key = li[1] / by;
+        if (dim == 3) key = (int32_t)((li[2] / bz) * nby + li[1] / by);
+        key_out[j] = key;
+    }
+    return bad ? 3 : 0;
+}
+
+/* Stable counting sort: perm[k] = index of the k-th point in (key, index)   */
+/* order.  offsets[b] = first sorted slot of key b, offsets[nkeys] = M.       */
+/* Restates the stable species counting sort of mp_lbin56.f90:1336-1347 and   */
+/* is the definition of the bin-sort permutation (a4.1).                     */
+int orc_counting_sort(int64_t M, const int32_t *key, int32_t nkeys,
+                      int32_t *perm, int64_t *offsets)
+{
+    if (nkeys < 1) return 2;
+    int64_t *cnt = (int64_t *)calloc((size_t)nkeys + 1, sizeof(int64_t));
+    if (!cnt) return 2;
+    for (int64_t j = 0; j < M; ++j) {
+        if (key[j] < 0 || key[j] >= nkeys) { free(cnt); return 2; }
+        cnt[key[j] + 1]++;
+    }
+    for (int32_t b = 0; b < nkeys; ++b) cnt[b + 1] += cnt[b];
+    if (offsets)
+        for (int32_t b = 0; b <= nkeys; ++b) offsets[b] = cnt[b];
+    for (int64_t j = 0; j < M; ++j) perm[cnt[key[j]]++] = (int32_t)j;
+    free(cnt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a2: point selection + compaction (order preserved).                       */
+/* CELL (mp_sqom56.f90:1104-1113): keep site i iff its position is not the   */
+/* vacancy marker (0,0,0) and every cell index lies in the real-space window */
+/* (ind[d] - n_row[d]/2 - 1) in [pmin[d], pmax[d]].                          */
+/* ------------------------------------------------------------------------ */
+int64_t orc_select_cell(int64_t nsuper, const float *pos, const int32_t *ind,
+                        const int32_t *n_row, const float *pmin,
+                        const float *pmax, int32_t *sel)
+{
+    int64_t k = 0;
+    for (int64_t i = 0; i < nsuper; ++i) {
+        const float *p = pos + 4 * i;
+        const int nz = (p[0] != 0.0f) || (p[1] != 0.0f) || (p[2] != 0.0f);
+        int inside = 1;
+        for (int d = 0; d < 3; ++d) {
+            const float v = (float)(ind[4 * i + d] - n_row[d] / 2 - 1);
+            inside = inside && (v >= pmin[d]) && (v <= pmax[d]);
+        }
+        if (nz && inside) sel[k++] = (int32_t)i;
+    }
+    return k;
+}
+
+/* BULK (mp_sqom56.f90:1121-1132): keep atom i iff pmin-m <= r < pmax+m in    */
+/* every direction, with margin m = eps_x (0.1, :214) for a single box        */
+/* (nsuper==1) else 1.  The bounds are formed in FP32 as in the reference.    */
+int64_t orc_select_bulk(int64_t n, const float *pos, const float *pmin,
+                        const float *pmax, float margin, int32_t *sel)
+{
+    float lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = pmin[d] - margin;
+        hi[d] = pmax[d] + margin;
+    }
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const float *p = pos + 4 * i;
+        int inside = 1;
+        for (int d = 0; d < 3; ++d) inside = inside && (p[d] >= lo[d]) && (p[d] < hi[d]);
+        if (inside) sel[k++] = (int32_t)i;
+    }
+    return k;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a3: staging of NUFFT coordinates and strengths (mp_sqom56.f90:1115-1118,  */
+/* 1134-1137, 809-811, 1167).  For every selected atom with FP32 position r: */
+/*   x_d = ((p0*e_d0 + p1*e_d1) + p2*e_d2) * tpe_d,  p = r - centre (FP64)    */
+/*   c   = weight * exp(-i * 2pi * (q0 . r))                                 */
+/*   c  *= prod_d (1 + cos x_d) / 2   if window (the reference's             */
+/*         (1+cos x)(1+cos y) with its 0.25 norm, :1118,:1167)               */
+/* axes e is 3x3 (row d = plane vector d, already normalised).               */
+/* ------------------------------------------------------------------------ */
+void orc_stage(int dim, int64_t M, const float *pos, const int32_t *sel,
+               const double *centre, const double *e, const double *tpe,
+               const double *q0, int window, double weight, double *x,
+               double *y, double *z, double *c)
+{
+    double *out[3] = {x, y, z};
+#pragma omp parallel for
+    for (int64_t j = 0; j < M; ++j) {
+        const int64_t i = sel ? sel[j] : j;
+        const double r0 = (double)pos[4 * i], r1 = (double)pos[4 * i + 1], r2 = (double)pos[4 * i + 2];
+        const double p0 = r0 - centre[0], p1 = r1 - centre[1], p2 = r2 - centre[2];
+        double xv[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            const double a = p0 * e[3 * d];
+            const double b = p1 * e[3 * d + 1];
+            const double cc = p2 * e[3 * d + 2];
+            const double dot = (a + b) + cc;
+            xv[d] = dot * tpe[d];
+            out[d][j] = xv[d];
+        }
+        const double ph = ORC_TWO_PI * ((q0[0] * r0 + q0[1] * r1) + q0[2] * r2);
+        double cr = weight * cos(ph), ci = -weight * sin(ph);
+        if (window) {
+            double wf = 1.0;
+            for (int d = 0; d < dim; ++d) wf *= 0.5 * (1.0 + cos(xv[d]));
+            cr *= wf;
+            ci *= wf;
+        }
+        c[2 * j] = cr;
+        c[2 * j + 1] = ci;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a7-a14: the S(Q) / S(Q,w) reduction, FP64, by direct windowed DFT.        */
+/* F   : per-species amplitudes [T][nsp][NQ] interleaved complex.            */
+/* wsc : per-species scalar weights [nsp] (used when wtab == NULL)          */
+/* wtab: per-species per-Q weights [nsp][NQ] or NULL                        */
+/* mask: [nsp] 0/1                                                          */
+/* qfac: per-Q real factor applied to the E-resolved amplitude or NULL      */
+/*       (q2_norm when j_qsq != 1, mp_sqom56.f90:1376, 901-917)              */
+/* Outputs:                                                                 */
+/*   stot[Q] = sum_t |C(t,Q)|^2 / (T * nat^2)          (:1381-1387, :1574)   */
+/*      with C = sum_j m_j w_j F_j                      (:1338-1356)         */
+/*   s[mi][Q], m = m_lo + mi, mi < n_f:                                     */
+/*      C'(t,Q) = [sum_j m_j w_j (F_j - M_j)] * qfac    (:1358-1377)         */
+/*      M_j(Q) = (1/T) sum_t F_j(t,Q)                   (:1177-1185)         */
+/*      X_s(m) = sum_i win_i exp(-2 pi i m i / n_int) C'(s+i)  (:1437-1466;  */
+/*               :1480-1501 computes the same |X| by FFT)                    */
+/*      s = dt / (nat^2 * 0.5 * n_frame * n_int) * sum_s |X_s|^2  (:1527,    */
+/*          :1576), n_frame = T - n_int + 1                                  */
+/*   window: kind 0 = 0.5(1 - cos(2 pi i/(n_int-1))), i = 0..n_int-1 (:941-948) */
+/*           kind 1 = 0.5(1 - cos(2 pi (i+1)/(n_int+1)))  (:2429-2431)       */
+/*           n_int <= 1 -> window 1                                          */
+/* subtract_mean = 0 skips the M_j term (the pair mode raw sums, :1460-1463) */
+/* ------------------------------------------------------------------------ */
+int orc_sqw_direct(int64_t T, int64_t NQ, int nsp, const double *F,
+                   const double *wsc, const double *wtab, const int32_t *mask,
+                   const double *qfac, int64_t n_int, int win_kind, double dt,
+                   double nat, int64_t m_lo, int64_t n_f, int subtract_mean,
+                   double *s, double *stot)
+{
+    if (T < 1 || NQ < 0 || nsp < 1 || n_int < 1 || n_int > T || n_f < 0) return 2;
+    const int64_t n_frame = T - n_int + 1;
+    double *win = (double *)malloc((size_t)n_int * sizeof(double));
+    double *twr = (double *)malloc((size_t)(n_f * n_int) * sizeof(double));
+    double *twi = (double *)malloc((size_t)(n_f * n_int) * sizeof(double));
+    for (int64_t i = 0; i < n_int; ++i) {
+        if (n_int <= 1) win[i] = 1.0;
+        else if (win_kind == 0) win[i] = 0.5 * (1.0 - cos(ORC_TWO_PI * (double)i / (double)(n_int - 1)));
+        else win[i] = 0.5 * (1.0 - cos(ORC_TWO_PI * (double)(i + 1) / (double)(n_int + 1)));
+    }
+    for (int64_t mi = 0; mi < n_f; ++mi) {
+        const int64_t m = m_lo + mi;
+        for (int64_t i = 0; i < n_int; ++i) {
+            int64_t p = (m * i) % n_int;
+            if (p < 0) p += n_int;
+            const double ph = -ORC_TWO_PI * (double)p / (double)n_int;
+            twr[mi * n_int + i] = win[i] * cos(ph);
+            twi[mi * n_int + i] = win[i] * sin(ph);
+        }
+    }
+    const double inv_n2 = 1.0 / (nat * nat);
+    const double norm_w = dt * inv_n2 / (0.5 * (double)n_frame * (double)n_int);
+#pragma omp parallel
+    {
+        double *cc = (double *)malloc((size_t)(2 * T) * sizeof(double));
+        double *cp = (double *)malloc((size_t)(2 * T) * sizeof(double));
+#pragma omp for schedule(static)
+        for (int64_t q = 0; q < NQ; ++q) {
+            double mean[2 * 64];
+            double *mj = (nsp <= 64) ? mean : (double *)malloc((size_t)(2 * nsp) * sizeof(double));
+            for (int j = 0; j < nsp; ++j) {
+                double ar = 0.0, ai = 0.0;
+                for (int64_t t = 0; t < T; ++t) {
+                    ar += F[2 * ((t * nsp + j) * NQ + q)];
+                    ai += F[2 * ((t * nsp + j) * NQ + q) + 1];
+                }
+                mj[2 * j] = ar / (double)T;
+                mj[2 * j + 1] = ai / (double)T;
+            }
+            double tot = 0.0;
+            for (int64_t t = 0; t < T; ++t) {
+                double sr = 0.0, si = 0.0, pr = 0.0, pi = 0.0;
+                for (int j = 0; j < nsp; ++j) {
+                    if (!mask[j]) continue;
+                    const double wv = wtab ? wtab[(int64_t)j * NQ + q] : wsc[j];
+                    const double fr = F[2 * ((t * nsp + j) * NQ + q)];
+                    const double fi = F[2 * ((t * nsp + j) * NQ + q) + 1];
+                    sr += wv * fr;
+                    si += wv * fi;
+                    if (subtract_mean) {
+                        pr += wv * (fr - mj[2 * j]);
+                        pi += wv * (fi - mj[2 * j + 1]);
+                    } else {
+                        pr += wv * fr;
+                        pi += wv * fi;
+                    }
+                }
+                const double qf = qfac ? qfac[q] : 1.0;
+                cc[2 * t] = sr;
+                cc[2 * t + 1] = si;
+                cp[2 * t] = pr * qf;
+                cp[2 * t + 1] = pi * qf;
+                tot += sr * sr + si * si;
+            }
+            if (stot) stot[q] = tot / (double)T * inv_n2;
+            for (int64_t mi = 0; mi < n_f; ++mi) {
+                double acc = 0.0;
+                for (int64_t sidx = 0; sidx < n_frame; ++sidx) {
+                    double xr = 0.0, xi = 0.0;
+                    for (int64_t i = 0; i < n_int; ++i) {
+                        const double ar = cp[2 * (sidx + i)], ai = cp[2 * (sidx + i) + 1];
+                        const double br = twr[mi * n_int + i], bi = twi[mi * n_int + i];
+                        xr += ar * br - ai * bi;
+                        xi += ar * bi + ai * br;
+                    }
+                    acc += xr * xr + xi * xi;
+                }
+                s[mi * NQ + q] = acc * norm_w;
+            }
+            if (mj != mean) free(mj);
+        }
+        free(cc);
+        free(cp);
+    }
+    free(win);
+    free(twr);
+    free(twi);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4.2: exponential-of-semicircle spreading onto the periodic fine grid,    */
+/* FP64, OpenMP (the FINUFFT method, barnett_FINUFFT.txt:38-40).  Used by    */
+/* the CPU FINUFFT-method restatement that bench.py times beside the GPU.    */
+/*   phi(t) = exp(beta * (sqrt(1 - (2t/w)^2) - 1)),  |t| <= w/2              */
+/*   grid[l] += c_j * prod_d phi(l_d - u_d)  for the w^dim cells of point j  */
+/* The grid is partitioned into slabs along the slowest used dimension; each */
+/* thread owns a slab and walks the points whose kernel rows touch it, so no */
+/* two threads write one cell.  grid is interleaved complex [n3][n2][n1].    */
+/* ------------------------------------------------------------------------ */
+int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
+                  const double *z, const double *c, const int64_t *n, int w,
+                  double beta, double *grid)
+{
+    if (dim < 1 || dim > 3 || w < 2 || w > 16) return 2;
+    const int64_t n1 = n[0], n2 = dim >= 2 ? n[1] : 1, n3 = dim >= 3 ? n[2] : 1;
+    const int64_t tot = n1 * n2 * n3;
+    memset(grid, 0, (size_t)(2 * tot) * sizeof(double));
+    int32_t *l = (int32_t *)malloc((size_t)(3 * M) * sizeof(int32_t));
+    double *d = (double *)malloc((size_t)(3 * M) * sizeof(double));
+    int32_t *key = (int32_t *)malloc((size_t)M * sizeof(int32_t));
+    const int64_t nn[3] = {n1, n2, n3};
+    orc_fold(dim, M, x, y, z, nn, w, 1, 1, l, d, key);
+    const int sd = dim - 1;               /* slab dimension */
+    const int64_t ns = nn[sd];
+    /* bucket points by first kernel cell along the slab dimension */
+    int64_t *start = (int64_t *)calloc((size_t)ns + 1, sizeof(int64_t));
+    int32_t *ord = (int32_t *)malloc((size_t)M * sizeof(int32_t));
+    for (int64_t j = 0; j < M; ++j) start[l[3 * j + sd] + 1]++;
+    for (int64_t i = 0; i < ns; ++i) start[i + 1] += start[i];
+    {
+        int64_t *fill = (int64_t *)malloc((size_t)ns * sizeof(int64_t));
+        memcpy(fill, start, (size_t)ns * sizeof(int64_t));
+        for (int64_t j = 0; j < M; ++j) ord[fill[l[3 * j + sd]]++] = (int32_t)j;
+        free(fill);
+    }
+    const double hw = 0.5 * w, ic = 1.0 / (hw * hw);
+#pragma omp parallel
+    {
+        int nt = 1, tid = 0;
+#ifdef _OPENMP
+        nt = omp_get_num_threads();
+        tid = omp_get_thread_num();
+#endif
+        const int64_t s0 = ns * tid / nt, s1 = ns * (tid + 1) / nt;
+        double ker[3][16];
+        if (s1 > s0) {
+            /* first-cell values whose rows reach [s0, s1): l in [s0-w+1, s1-1];
+             * when that range wraps onto itself every bucket is visited once */
+            const int64_t len = (s1 - s0) + w - 1;
+            const int64_t lv0 = (len >= ns) ? 0 : s0 - w + 1;
+            const int64_t lv1 = (len >= ns) ? ns : s1;
+            for (int64_t lv = lv0; lv < lv1; ++lv) {
+                const int64_t lb = ((lv % ns) + ns) % ns;
+                for (int64_t q = start[lb]; q < start[lb + 1]; ++q) {
+                    const int64_t j = ord[q];
+                    for (int dd = 0; dd < 3; ++dd)
+                        for (int m = 0; m < w; ++m) {
+                            if (dd < dim) {
+                                const double t = d[3 * j + dd] + (double)m;
+                                const double a = 1.0 - t * t * ic;
+                                ker[dd][m] = a > 0.0 ? exp(beta * (sqrt(a) - 1.0)) : 0.0;
+                            } else {
+                                ker[dd][m] = (m == 0) ? 1.0 : 0.0;
+                            }
+                        }
+                    const double cr = c[2 * j], ci = c[2 * j + 1];
+                    const int w2 = dim >= 2 ? w : 1, w3 = dim >= 3 ? w : 1;
+                    for (int m3 = 0; m3 < w3; ++m3) {
+                        const int64_t i3 = (l[3 * j + 2] + m3) % n3;
+                        if (sd == 2 && (i3 < s0 || i3 >= s1)) continue;
+                        for (int m2 = 0; m2 < w2; ++m2) {
+                            const int64_t i2 = (l[3 * j + 1] + m2) % n2;
+                            if (sd == 1 && (i2 < s0 || i2 >= s1)) continue;
+                            const double k23 = ker[1][m2] * ker[2][m3];
+                            double *row = grid + 2 * ((i3 * n2 + i2) * n1);
+                            for (int m1 = 0; m1 < w; ++m1) {
+                                const int64_t i1 = (l[3 * j] + m1) % n1;
+                                if (sd == 0 && (i1 < s0 || i1 >= s1)) continue;
+                                const double kv = ker[0][m1] * k23;
+                                row[2 * i1] += cr * kv;
+                                row[2 * i1 + 1] += ci * kv;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(l); free(d); free(key); free(start); free(ord);
+    return 0;
+}

@@ -1,0 +1,264 @@
+"""Pinning the CPU oracle (runs on the CPU; no GPU).
+
+Each check is chosen so that a plausible mistake in the oracle (sign, index
+offset, transposed axis, dropped term) fails it: closed forms, textbook
+library routines (numpy.fft, numpy's stable argsort, scipy quadrature) and
+invariants.  The reference has no test suite (its only built-in check is the
+direct transform dirft2d1, code/src/mp_sqom56.f90:2530-2602, whose
+definition orc_dirft_type1 restates).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------- direct DFT
+@pytest.mark.parametrize("shape", [(16,), (12, 10), (8, 6, 4)])
+def test_dirft_uniform_grid_is_fft(shape):
+    """Points on the uniform grid x = 2 pi j / n: the type-1 sum is numpy's
+    FFT (sign -1) sampled at k mod n."""
+    rng = np.random.default_rng(1)
+    dim = len(shape)
+    n = shape
+    grids = np.meshgrid(*[2 * np.pi * np.arange(v) / v for v in n], indexing="ij")
+    c = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    co = [g.ravel() for g in grids]
+    F = O.dirft_type1(*(co + [None] * (3 - dim)), c.ravel(), n, iflag=-1)
+    ref = np.fft.fftn(c)
+    ks = [np.arange(-(v // 2), (v - 1) // 2 + 1) for v in n]
+    ref = ref[np.ix_(*[np.mod(k, v) for k, v in zip(ks, n)])]
+    # oracle layout is reversed(n) (k1 fastest)
+    assert np.allclose(F, np.transpose(ref, tuple(reversed(range(dim)))), atol=1e-10)
+
+
+def test_dirft_single_point_origin_and_shift():
+    rng = np.random.default_rng(2)
+    nm = (9, 8, 6)
+    c0 = np.array([0.3 - 1.2j])
+    F = O.dirft_type1([0.0], [0.0], [0.0], c0, nm)
+    assert np.allclose(F, c0[0], atol=1e-14)
+    M = 50
+    x, y, z = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    d = (0.37, -1.1, 2.0)
+    F0 = O.dirft_type1(x, y, z, c, nm)
+    F1 = O.dirft_type1(x + d[0], y + d[1], z + d[2], c, nm)
+    k1 = np.arange(-(nm[0] // 2), (nm[0] - 1) // 2 + 1)
+    k2 = np.arange(-(nm[1] // 2), (nm[1] - 1) // 2 + 1)
+    k3 = np.arange(-(nm[2] // 2), (nm[2] - 1) // 2 + 1)
+    ph = np.exp(-1j * (k3[:, None, None] * d[2] + k2[None, :, None] * d[1] + k1[None, None, :] * d[0]))
+    assert np.allclose(F1, F0 * ph, atol=1e-10)
+
+
+def test_dirft_perfect_lattice_bragg_pattern():
+    """L^3 sites x = 2 pi j / L: F = L^3 at k = 0 mod L, 0 elsewhere."""
+    L = 4
+    g = np.meshgrid(*[2 * np.pi * np.arange(L) / L - np.pi] * 3, indexing="ij")
+    co = [v.ravel() for v in g]
+    nm = (12, 10, 8)
+    F = O.dirft_type1(*co, np.ones(L ** 3), nm)
+    ks = [np.arange(-(v // 2), (v - 1) // 2 + 1) for v in nm]
+    K3, K2, K1 = np.meshgrid(ks[2], ks[1], ks[0], indexing="ij")
+    bragg = (K1 % L == 0) & (K2 % L == 0) & (K3 % L == 0)
+    sign = (-1.0) ** ((K1 + K2 + K3) // L * L)   # the -pi offset: exp(i k pi)
+    assert np.allclose(np.abs(F[bragg]), L ** 3, atol=1e-9)
+    assert np.allclose(F[~bragg], 0, atol=1e-9)
+    assert np.allclose(F[bragg], (L ** 3) * sign[bragg], atol=1e-9)
+
+
+def test_dirft_linearity_sign_and_slices():
+    rng = np.random.default_rng(3)
+    M, nm = 60, (7, 6, 5)
+    x, y, z = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    a = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    b = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    Fa = O.dirft_type1(x, y, z, a, nm)
+    Fb = O.dirft_type1(x, y, z, b, nm)
+    Fab = O.dirft_type1(x, y, z, 2.0 * a - 0.5j * b, nm)
+    assert np.allclose(Fab, 2.0 * Fa - 0.5j * Fb, atol=1e-11)
+    # iflag +1 with conj(c) is conj of iflag -1
+    Fp = O.dirft_type1(x, y, z, np.conj(a), nm, iflag=+1)
+    assert np.allclose(Fp, np.conj(Fa), atol=1e-11)
+    # 3D = stack of 2D slices with strengths c exp(-i k3 z) (mp_sql56.f90:1110-1132)
+    for i3, k3 in enumerate(range(-(nm[2] // 2), (nm[2] - 1) // 2 + 1)):
+        F2 = O.dirft_type1(x, y, None, a * np.exp(-1j * k3 * z), nm[:2])
+        assert np.allclose(F2, Fa[i3], atol=1e-11)
+
+
+def test_dirft_sampled_matches_dense_and_bruteforce():
+    rng = np.random.default_rng(4)
+    M, nm = 700, (16, 12, 10)
+    x, y, z = [rng.uniform(-3 * np.pi, 3 * np.pi, M) for _ in range(3)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    F = O.dirft_type1(x, y, z, c, nm)
+    K = 200
+    k1 = rng.integers(-8, 8, K)
+    k2 = rng.integers(-6, 6, K)
+    k3 = rng.integers(-5, 5, K)
+    Fs = O.dirft_type1_sampled(x, y, z, c, k1, k2, k3)
+    assert np.allclose(Fs, F[k3 + 5, k2 + 6, k1 + 8], atol=1e-9)
+    brute = np.array([np.sum(c * np.exp(-1j * (a * x + b * y + d * z))) for a, b, d in zip(k1, k2, k3)])
+    assert np.allclose(Fs, brute, atol=1e-9)
+
+
+# ---------------------------------------------------------------- fold + sort
+def test_fold_properties():
+    rng = np.random.default_rng(5)
+    M, nf, w = 5000, (64, 48, 32), 7
+    x, y, z = [rng.uniform(-5 * np.pi, 5 * np.pi, M) for _ in range(3)]
+    x[:4] = [-np.pi, np.pi, 0.0, -3 * np.pi]
+    l, d, key, rc = O.fold(x, y, z, nf, w, by=4, bz=2)
+    assert rc == 0
+    for k, (co, n) in enumerate(zip((x, y, z), nf)):
+        assert np.all((l[:, k] >= 0) & (l[:, k] < n))
+        assert np.all((d[:, k] >= -w / 2) & (d[:, k] < -w / 2 + 1))
+        u = co * n / (2 * np.pi)
+        # the point sits at grid position l - d (mod n)
+        diff = np.mod(l[:, k] - d[:, k] - u + n / 2, n) - n / 2
+        assert np.max(np.abs(diff)) < 1e-9
+    assert np.array_equal(key, (l[:, 2] // 2) * (48 // 4) + l[:, 1] // 4)
+    # a shift by 2 pi leaves the cell indices unchanged
+    l2, d2, key2, _ = O.fold(x + 2 * np.pi, y, z, nf, w, by=4, bz=2)
+    same = np.abs(d2[:, 0] - d[:, 0]) < 1e-6
+    assert same.mean() > 0.99 and np.array_equal(l2[same, 0], l[same, 0])
+    # non-finite -> error code
+    x[3] = np.inf
+    assert O.fold(x, y, z, nf, w)[3] != 0
+
+
+def test_counting_sort_is_numpy_stable_argsort():
+    rng = np.random.default_rng(6)
+    key = rng.integers(0, 97, 20000).astype(np.int32)
+    perm, off = O.counting_sort(key, 97)
+    assert np.array_equal(perm, np.argsort(key, kind="stable"))
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.bincount(key, minlength=97))]))
+
+
+# ---------------------------------------------------------------- staging / selection
+def test_stage_closed_forms():
+    # integer lattice sites and integer q0: the phase exp(-i 2 pi q0.r) is 1
+    pos = np.zeros((5, 4), np.float32)
+    pos[:, :3] = np.array([[0, 0, 0], [1, 2, 3], [-4, 1, 0], [2, 2, 2], [-1, -1, 5]])
+    x, y, z, c = O.stage(pos, None, (0, 0, 0), np.eye(3), (0.5, 0.25, 0.125), (2, -1, 3), 0, 1.0)
+    assert np.allclose(c, 1.0, atol=1e-12)
+    assert np.allclose(x, pos[:, 0] * 0.5) and np.allclose(z, pos[:, 2] * 0.125)
+    # half-integer q0 on odd sites flips the sign
+    x, y, z, c = O.stage(pos, None, (0, 0, 0), np.eye(3), (1, 1, 1), (0.5, 0, 0), 0, 2.0)
+    assert np.allclose(c, 2.0 * (-1.0) ** pos[:, 0], atol=1e-12)
+    # spatial Hann window: 1 at x = 0, 0 at x = +-pi (2D form, mp_sqom56.f90:1118,1167)
+    p2 = np.zeros((3, 4), np.float32)
+    p2[:, 0] = [0.0, 1.0, -1.0]
+    x, y, z, c = O.stage(p2, None, (0, 0, 0), np.eye(3), (np.pi, np.pi, 1), (0, 0, 0), 1, 1.0, dim=2)
+    assert np.allclose(c, [1.0, 0.0, 0.0], atol=1e-7)
+    # rotated axes: x is the projection on e1 (mp_sqom56.f90:1115)
+    e = np.array([[1, 1, 0], [-1, 1, 0], [0, 0, 1]]) / np.array([[np.sqrt(2)], [np.sqrt(2)], [1]])
+    x, y, z, c = O.stage(pos, None, (0.5, 0, 0), e, (1, 1, 1), (0, 0, 0), 0, 1.0)
+    assert np.allclose(x, ((pos[:, 0] - 0.5) + pos[:, 1]) / np.sqrt(2), atol=1e-12)
+
+
+def test_selection_known_cases():
+    pos = np.zeros((6, 4), np.float32)
+    pos[:, :3] = [[0, 0, 0], [1, 1, 1], [0.5, 0, 0], [-3, 0, 0], [2.95, 0, 0], [3.1, 0, 0]]
+    ind = np.zeros((6, 4), np.int32)
+    ind[:, :3] = [[1, 1, 1], [2, 2, 2], [5, 5, 5], [6, 6, 6], [3, 3, 3], [1, 1, 1]]
+    # CELL: vacancy at (0,0,0) dropped; window on ind - n_row/2 - 1 (mp_sqom56.f90:1106-1113)
+    # ind - 5 = -4, -3, 0, 1, -2, -4 against the window [-4, 0]
+    sel = O.select_cell(pos, ind, (8, 8, 8), (-4, -4, -4), (0, 0, 0))
+    assert list(sel) == [1, 2, 4, 5]
+    # BULK: [pmin - m, pmax + m) with m = 0.1 (mp_sqom56.f90:1123-1126)
+    sel = O.select_bulk(pos, (-3, -3, -3), (3, 3, 3), 0.1)
+    assert list(sel) == [0, 1, 2, 3, 4]
+
+
+# ---------------------------------------------------------------- reduction
+def _hann(n, kind=0):
+    i = np.arange(n)
+    if n <= 1:
+        return np.ones(n)
+    if kind == 0:
+        return 0.5 * (1 - np.cos(2 * np.pi * i / (n - 1)))
+    return 0.5 * (1 - np.cos(2 * np.pi * (i + 1) / (n + 1)))
+
+
+def test_reduction_single_frame_and_constant_amplitudes():
+    rng = np.random.default_rng(7)
+    F = rng.standard_normal((1, 3, 40)) + 1j * rng.standard_normal((1, 3, 40))
+    w = np.array([0.5, 1.5, -0.3])
+    S, St = O.sqw_direct(F, wsc=w, n_int=1, nat=7.0)
+    assert np.allclose(St, np.abs((F[0] * w[:, None]).sum(0)) ** 2 / 49.0)
+    # time-constant amplitudes: the inelastic part vanishes (mean subtraction)
+    Fc = np.repeat(F, 16, axis=0)
+    S, St = O.sqw_direct(Fc, wsc=w, n_int=16, nat=7.0, n_f=16)
+    assert np.max(np.abs(S)) < 1e-20
+    assert np.allclose(St, np.abs((F[0] * w[:, None]).sum(0)) ** 2 / 49.0)
+
+
+@pytest.mark.parametrize("T,n_int,kind", [(32, 32, 0), (30, 11, 0), (25, 25, 1), (17, 6, 1)])
+def test_reduction_matches_numpy_fft(T, n_int, kind):
+    """sum over windows of |FFT(win * C')|^2 (numpy), normalised as
+    mp_sqom56.f90:1527,1576."""
+    rng = np.random.default_rng(T + n_int)
+    NQ = 9
+    F = rng.standard_normal((T, 2, NQ)) + 1j * rng.standard_normal((T, 2, NQ))
+    w = np.array([1.3, 0.4])
+    qf = rng.uniform(0.5, 2.0, NQ)
+    dt, nat = 0.02, 11.0
+    S, St = O.sqw_direct(F, wsc=w, qfac=qf, n_int=n_int, win_kind=kind, dt=dt, nat=nat,
+                         m_lo=-(n_int // 2), n_f=n_int)
+    C = (F * w[None, :, None]).sum(1)
+    Cp = (C - C.mean(0)) * qf
+    n_frame = T - n_int + 1
+    win = _hann(n_int, kind)
+    acc = np.zeros((n_int, NQ))
+    for s in range(n_frame):
+        acc += np.abs(np.fft.fft(win[:, None] * Cp[s:s + n_int], axis=0)) ** 2
+    m = np.mod(np.arange(-(n_int // 2), -(n_int // 2) + n_int), n_int)
+    ref = acc[m] * dt / (nat ** 2 * 0.5 * n_frame * n_int)
+    assert rel_l2(S, ref) < 1e-12
+    assert np.allclose(St, (np.abs(C) ** 2).sum(0) / (T * nat ** 2))
+    # Parseval over all n_int frequencies
+    pars = sum(np.sum((win[:, None] * np.abs(Cp[s:s + n_int])) ** 2, axis=0) for s in range(n_frame))
+    assert np.allclose(S.sum(0), n_int * pars * dt / (nat ** 2 * 0.5 * n_frame * n_int))
+
+
+def test_reduction_pure_tone_peak():
+    T, m0 = 64, 5
+    t = np.arange(T)
+    F = np.exp(-2j * np.pi * m0 * t / T)[:, None, None] * np.ones((1, 1, 3))
+    F += 2.0  # elastic part, removed by the mean subtraction
+    S, _ = O.sqw_direct(F, n_int=T, n_f=T)
+    assert np.all(np.argmax(S, axis=0) == (T - m0) % T)
+
+
+# ---------------------------------------------------------------- FINUFFT method on the CPU
+def test_es_phihat_against_scipy_quad():
+    from scipy.integrate import quad
+    w, beta = O.es_params(1e-6)
+    n = 64
+    hw = w / 2
+    for k in (0, 5, 16, 31):
+        f = lambda t: np.exp(beta * (np.sqrt(max(0.0, 1 - (t / hw) ** 2)) - 1)) * np.cos(2 * np.pi * k * t / n)
+        ref = 2 * quad(f, 0, hw, limit=200, epsabs=1e-14, epsrel=1e-13)[0]
+        assert abs(O.es_phihat([k], n, w, beta)[0] - ref) < 1e-9 * abs(ref)
+
+
+@pytest.mark.parametrize("eps", [1e-3, 1e-6, 1e-9])
+def test_cpu_nufft_method_reaches_eps(eps):
+    rng = np.random.default_rng(8)
+    M, nm = 1500, (20, 16, 10)
+    x, y, z = [rng.uniform(-np.pi, np.pi, M) for _ in range(3)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    F = O.dirft_type1(x, y, z, c, nm)
+    G = O.cpu_nufft_type1(x, y, z, c, nm, eps=eps)
+    assert rel_l2(G, F) < 5 * eps
+
+
+def test_es_params_published_rule():
+    assert O.es_params(1e-6)[0] == 7
+    assert O.es_params(1e-3)[0] == 4
+    assert O.es_params(1e-9)[0] == 10
+    assert abs(O.es_params(1e-6)[1] - 2.30 * 7) < 1e-12
+    assert O.fine_size(256, 7) == 512 and O.fine_size(32, 7) == 64 and O.fine_size(160, 7) == 320
+    assert O.next_smooth_even(1018) == 1024 and O.next_smooth_even(14) == 16
