@@ -1,0 +1,246 @@
+"""S(Q,w) benchmark on one MI355X.
+
+Workload (the configuration the metric is quoted on): cubic ABO3 perovskite,
+40^3 cells = 320,000 atoms in 3 species, 512 MD-like frames at dt = 10 fs
+(64 commensurate plane-wave phonons, rms 0.06 A, plus 0.01 A noise), a
+per-species type-1 NUFFT to 256 x 256 x 32 modes at eps 1e-6, species combine
+with neutron weights, one Hann window over the 512 frames (n_int = 512,
+n_frame = 1), time FFT and |.|^2 -> S(Q,w) for m = 0..255 plus S(Q).
+
+A step is one full pass: 512 frames x 3 species NUFFTs + combine + reduction,
+inputs resident in HBM.  value = frames/s of the FP32 form (FP32 spread /
+FFT / deconvolution, FP64 fold and FP64 final sums).  The plain FP64 form
+runs beside it on the same trajectory and is the reference for the FP32
+form's error; the CPU leg times the host FINUFFT-method restatement
+(oracle/) on a bounded sample of the same frames.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames T]
+                       [--no-fp64] [--no-cpu] [--cpu-frames F]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--frames", type=int, default=512)
+    ap.add_argument("--ncell", type=int, default=40)
+    ap.add_argument("--modes", type=str, default="256,256,32")
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=20261204)
+    return ap.parse_args()
+
+
+def nufft_bytes(M, N, n, s_c):
+    """Survey model of one type-1 transform: points + strengths once, fine
+    grid written by spread and read + written by the FFT, modes r/w."""
+    return M * (12 + s_c) + 3 * s_c * math.prod(n) + 2 * s_c * math.prod(N)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported "
+                                   "(the multi-GPU sharding is a design only)"}))
+        sys.exit(2)
+    import torch
+
+    from mp_tools_amd import synth
+    from mp_tools_amd.sqw import SqwEngine
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda:0")
+    N = tuple(int(v) for v in a.modes.split(","))
+    NQ = math.prod(N)
+    T, ncell = a.frames, a.ncell
+    t0 = time.time()
+    pos, counts = synth.phonon_trajectory(ncell, T, dt_ps=0.01, n_phonons=64, rms=0.06,
+                                          noise=0.01, seed=a.seed, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    M = int(counts.sum())
+    nat = float(M)
+    w = 0.1 * synth.B_NEUTRON_FM                      # b_coh * 0.1 (mp_sqom56.f90:473)
+    geo = synth.cell_geometry(ncell)
+    n_f = T // 2
+    dt = 0.01
+
+    def make(precision):
+        return SqwEngine(3, N, counts, eps=a.eps, precision=precision, weights=w,
+                         tpe=geo["tpe"], batch_frames=a.batch)
+
+    def step(eng, out):
+        amp = eng.amplitudes(pos, out=out["amp"])
+        eng.reduce(amp, n_int=T, win_kind=0, dt=dt, nat=nat, m_lo=0, n_f=n_f,
+                   out=out["S"], out_tot=out["St"])
+
+    def run(precision, steps, warmup):
+        eng = make(precision)
+        cdt = torch.complex128 if precision == "fp64" else torch.complex64
+        out = dict(amp=torch.empty((T, NQ), dtype=cdt, device=dev),
+                   S=torch.empty((n_f, NQ), dtype=torch.float64, device=dev),
+                   St=torch.empty(NQ, dtype=torch.float64, device=dev))
+        for _ in range(warmup):
+            step(eng, out)
+        torch.cuda.synchronize()
+        eng.set_timing(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            step(eng, out)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        tm = eng.timing()
+        return eng, out, el / steps, tm
+
+    eng32, out32, sec32, tm32 = run("fp32", a.steps, a.warmup)
+    frames_s = T / sec32
+    res = {
+        "metric": "S(Q,w) frames/s (FP32 form, rel.err vs FP64)",
+        "value": frames_s,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": sec32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded perovskite phonon trajectory generated on the GPU)",
+        "config": {"workload": f"perovskite {ncell}^3 cells ({M} atoms, 3 species) x {T} frames, "
+                               f"per-species type-1 NUFFT to {N[0]}x{N[1]}x{N[2]} modes eps {a.eps:g}, "
+                               f"neutron-weighted combine, Hann time FFT n_int={T}, S(Q,w) m=0..{n_f - 1}",
+                   "frames": T, "atoms": M, "species": 3, "modes": list(N), "eps": a.eps,
+                   "n_int": T, "n_f": n_f, "batch_frames": eng32.__dict__.get("batch", a.batch),
+                   "seed": a.seed},
+        "nu_pts_per_s": T * M / sec32,
+        "trajectory_gen_s": gen_s,
+    }
+    # stage times (HIP events on the engine's stream over the timed steps)
+    stages = {}
+    for k, (ms, n) in tm32.items():
+        if n:
+            stages[k] = {"ms_per_step": ms / a.steps, "launches_per_step": n / a.steps,
+                         "avg_ms_per_launch": ms / n}
+    res["stages_fp32"] = stages
+    # roofline of the dominant kernel (per-kernel algorithmic bytes)
+    n_fine = (2 * N[0], 2 * N[1], 2 * N[2])
+    s_c = 8
+    alg = {
+        "spread_xfft": T * (M * (12 + s_c) + 3 * s_c * n_fine[2] * n_fine[1] * N[0]),
+        "colfft": T * 3 * s_c * (n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
+        "final_colfft": T * s_c * (3 * n_fine[2] * N[1] * N[0] + NQ),
+        "reduce": T * NQ * s_c + NQ * (n_f + 1) * 8,
+        "sort": T * M * (4 + 2 * 32 + 2 * s_c),
+        "stage": T * M * (16 + 32 + 4 + s_c),
+    }
+    dom = max(stages, key=lambda k: stages[k]["ms_per_step"])
+    launches = stages[dom]["launches_per_step"]
+    per_launch_bytes = alg.get(dom, 0) / launches
+    achieved = per_launch_bytes / (stages[dom]["avg_ms_per_launch"] * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2_fp32.json")
+    if os.path.exists(tfile):
+        try:
+            traffic = json.load(open(tfile)).get(dom)
+        except Exception:
+            traffic = None
+    res["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                       "algorithmic_bytes_per_launch": per_launch_bytes}
+    for k in stages:
+        if k in alg:
+            stages[k]["alg_GBps"] = alg[k] / (stages[k]["ms_per_step"] * 1e-3) / 1e9
+    survey_bytes = T * 3 * 0  # per-species model below
+    sp_counts = [int(v) for v in counts]
+    survey_bytes = sum(T * nufft_bytes(m, N, n_fine, s_c) for m in sp_counts)
+    survey_bytes += NQ * (3 * T * s_c + n_f * 8)
+    res["survey_model_GBps"] = survey_bytes / sec32 / 1e9
+
+    # plain FP64 form beside it, and the FP32 form's error against it
+    if not a.no_fp64:
+        eng64, out64, sec64, tm64 = run("fp64", max(1, min(a.steps, 2)), 1)
+        S32, S64 = out32["S"], out64["S"]
+        St32, St64 = out32["St"], out64["St"]
+        err_s = (torch.linalg.vector_norm(S32 - S64) / torch.linalg.vector_norm(S64)).item()
+        err_t = (torch.linalg.vector_norm(St32 - St64) / torch.linalg.vector_norm(St64)).item()
+        # the same without Bragg pixels (k_d multiple of ncell on the full grid)
+        ks = [torch.arange(-(v // 2), (v - 1) // 2 + 1, device=dev) for v in N]
+        K3, K2, K1 = torch.meshgrid(ks[2], ks[1], ks[0], indexing="ij")
+        bragg = ((K1 % ncell == 0) & (K2 % ncell == 0) & (K3 % ncell == 0)).reshape(-1)
+        keep = ~bragg
+        err_nb = (torch.linalg.vector_norm((S32 - S64)[:, keep]) /
+                  torch.linalg.vector_norm(S64[:, keep])).item()
+        res["fp64_frames_per_s"] = T / sec64
+        res["fp32_over_fp64_speedup"] = sec64 / sec32
+        res["rel_l2_S_fp32_vs_fp64"] = err_s
+        res["rel_l2_S_fp32_vs_fp64_no_bragg"] = err_nb
+        res["rel_l2_Stot_fp32_vs_fp64"] = err_t
+        res["stages_fp64"] = {k: {"ms_per_step": ms / max(1, min(a.steps, 2))}
+                              for k, (ms, n) in tm64.items() if n}
+        del out64, eng64
+        torch.cuda.empty_cache()
+
+    # CPU leg: the oracle's host FINUFFT-method restatement on a bounded sample
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_leg(a, pos, counts, w, N, out32["amp"], geo)
+    print(json.dumps(res))
+
+
+def cpu_leg(a, pos, counts, w, N, amp_gpu, geo):
+    """Host FINUFFT-method restatement (ES spreading in C/OpenMP, scipy.fft,
+    deconvolution) of the first F frames, all species, plus the CPU time
+    reduction of those frames; frames/s = F / wall."""
+    from oracle import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    O.set_threads(cores)
+    F = max(1, a.cpu_frames)
+    pn = pos[:F].cpu().numpy()
+    off = np.concatenate([[0], np.cumsum(counts)])
+    t = time.perf_counter()
+    C = np.zeros((F, int(np.prod(N))), np.complex128)
+    for f in range(F):
+        for j in range(len(counts)):
+            x, y, z, c = O.stage(pn[f, off[j]:off[j + 1]], None, (0, 0, 0), np.eye(3), geo["tpe"],
+                                 (0, 0, 0), 0, 1.0)
+            C[f] += w[j] * O.cpu_nufft_type1(x, y, z, c, N, eps=a.eps, workers=cores).ravel()
+    t_nufft = time.perf_counter() - t
+    t = time.perf_counter()
+    import scipy.fft as sfft
+    win = 0.5 * (1 - np.cos(2 * np.pi * np.arange(F) / max(F - 1, 1)))
+    Cp = (C - C.mean(0)) * win[:, None]
+    S = np.abs(sfft.fft(Cp, axis=0, workers=cores)) ** 2
+    t_red = time.perf_counter() - t
+    g = amp_gpu[:F].cpu().numpy().reshape(F, -1)
+    err = float(np.linalg.norm(g - C) / np.linalg.norm(C))
+    del S
+    return {"value": F / (t_nufft + t_red), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"first {F} of the {a.frames} frames, all 3 species: host FINUFFT-method "
+                      f"restatement (FP64 ES spreading, C/OpenMP; scipy.fft; deconvolution) + "
+                      f"species combine + windowed time FFT of those frames",
+            "wall_s": t_nufft + t_red, "nufft_s": t_nufft, "reduce_s": t_red,
+            "rel_l2_gpu_fp32_amplitudes_vs_cpu": err}
+
+
+if __name__ == "__main__":
+    main()

@@ -184,6 +184,13 @@ int mpsq_sqw_reduce(mpsq_sqw eng, const void *d_amp, int64_t T, const mpsq_reduc
 int mpsq_sqw_destroy(mpsq_sqw eng);
 /* Synchronise the engine's stream. */
 int mpsq_sqw_sync(mpsq_sqw eng);
+/* Per-stage timing with HIP events recorded on the engine's stream around
+ * every launch group.  Stages: 0 staging, 1 bin sort, 2 spread + x FFT,
+ * 3 non-final column FFT pass, 4 final column pass (deconvolve + combine),
+ * 5 S(Q,w) reduction.  set_timing(1) clears the accumulators;
+ * get_timing fills ms[k] (summed milliseconds) and launches[k] for k < n. */
+int mpsq_sqw_set_timing(mpsq_sqw eng, int on);
+int mpsq_sqw_get_timing(mpsq_sqw eng, double *ms, int64_t *launches, int n);
 /* Number of modes N1*N2*N3 and active species of an engine. */
 int64_t mpsq_sqw_nq(mpsq_sqw eng);
 int mpsq_sqw_nact(mpsq_sqw eng);

@@ -35,7 +35,9 @@ EXPORTS = [
     "mpsq_makeplan_", "mpsq_setpts_", "mpsq_execute_", "mpsq_destroy_",
     "mpsq_sqw_default_spec", "mpsq_sqw_create", "mpsq_sqw_amplitudes", "mpsq_sqw_reduce",
     "mpsq_sqw_destroy", "mpsq_sqw_sync", "mpsq_sqw_nq", "mpsq_sqw_nact", "mpsq_version",
+    "mpsq_sqw_set_timing", "mpsq_sqw_get_timing",
 ]
+STAGES = ("stage", "sort", "spread_xfft", "colfft", "final_colfft", "reduce")
 
 
 class MpsqError(RuntimeError):
@@ -124,6 +126,8 @@ def load():
     L.mpsq_sqw_nq.restype = i64
     L.mpsq_sqw_nact.argtypes = [vp]
     L.mpsq_version.restype = C.c_char_p
+    L.mpsq_sqw_set_timing.argtypes = [vp, i32]
+    L.mpsq_sqw_get_timing.argtypes = [vp, vp, vp, i32]
     _lib = L
     return L
 

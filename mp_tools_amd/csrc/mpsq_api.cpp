@@ -196,6 +196,63 @@ static int ilog2_ceil(int64_t v)
     return b;
 }
 
+// HIP-event timing of launch groups on one stream (stage categories of
+// mpsq_sqw_get_timing).  Events are pooled; elapsed times are collected
+// after the stream has been synchronised.
+struct StageTimer {
+    static constexpr int K = 6;
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> cat;
+    size_t used = 0;
+    double ms[K] = {};
+    int64_t n[K] = {};
+    void reset()
+    {
+        used = 0;
+        cat.clear();
+        for (int k = 0; k < K; ++k) {
+            ms[k] = 0.0;
+            n[k] = 0;
+        }
+    }
+    int begin(int k, hipStream_t st)
+    {
+        if (!on) return -1;
+        if (2 * used >= pool.size()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            pool.push_back(a);
+            pool.push_back(b);
+        }
+        cat.push_back(k);
+        HIPCHK(hipEventRecord(pool[2 * used], st));
+        return (int)used++;
+    }
+    void end(int idx, hipStream_t st)
+    {
+        if (idx >= 0) HIPCHK(hipEventRecord(pool[2 * (size_t)idx + 1], st));
+    }
+    void collect()
+    {
+        for (size_t i = 0; i < used; ++i) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, pool[2 * i], pool[2 * i + 1]));
+            ms[cat[i]] += t;
+            n[cat[i]] += 1;
+        }
+        used = 0;
+        cat.clear();
+    }
+    ~StageTimer()
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+enum { ST_STAGE = 0, ST_SORT = 1, ST_SPREAD = 2, ST_COLPASS = 3, ST_FINAL = 4, ST_REDUCE = 5 };
+
 // ---------------------------------------------------------------------------
 // geometry shared by plans and the S(Q,w) engine
 // ---------------------------------------------------------------------------
@@ -330,8 +387,10 @@ template <typename R> struct Core {
     // Launch the transform passes after spreading.  ntr transforms are laid
     // out back to back; FINAL groups nsp consecutive transforms.
     void passes(int ntr, int ngroups, int nsp, const Cx<R> *G1, Cx<R> *G2, Cx<R> *out, const R *wsc,
-                const float *wtab)
+                const float *wtab, StageTimer *tm = nullptr)
     {
+        StageTimer dummy;
+        if (!tm) tm = &dummy;
         const int conj = iflag > 0 ? 1 : 0;
         const int64_t NQ = N[0] * N[1] * N[2];
         if (dim == 2) {
@@ -346,7 +405,9 @@ template <typename R> struct Core {
             cp.nsp = nsp;
             cp.NQ = NQ;
             cp.conj = conj;
+            const int ev = tm->begin(ST_FINAL, st);
             launch_colfft<R>(cp, rp[1], ngroups, G1, out, tw[1].as<Cx<R>>(), phinv[1].as<R>(), wsc, wtab, true, st);
+            tm->end(ev, st);
         } else if (dim == 3) {
             ColPass cy{};
             cy.n = (int)n[1];
@@ -359,8 +420,10 @@ template <typename R> struct Core {
             cy.nsp = 1;
             cy.NQ = NQ;
             cy.conj = 0;
+            const int ev1 = tm->begin(ST_COLPASS, st);
             launch_colfft<R>(cy, rp[1], ntr, G1, G2, tw[1].as<Cx<R>>(), phinv[1].as<R>(), nullptr, nullptr, false,
                              st);
+            tm->end(ev1, st);
             ColPass cz{};
             cz.n = (int)n[2];
             cz.N = (int)N[2];
@@ -372,7 +435,9 @@ template <typename R> struct Core {
             cz.nsp = nsp;
             cz.NQ = NQ;
             cz.conj = conj;
+            const int ev2 = tm->begin(ST_FINAL, st);
             launch_colfft<R>(cz, rp[2], ngroups, G2, out, tw[2].as<Cx<R>>(), phinv[2].as<R>(), wsc, wtab, true, st);
+            tm->end(ev2, st);
         }
     }
 
@@ -616,6 +681,7 @@ template <typename R> struct Engine {
     std::vector<TransformInfo> tinfo;
     // reduction caches
     DevBuf win_d, twT;
+    StageTimer tm;
     std::vector<double> win_h;
     std::vector<Cx<R>> twT_h;
     int win_n = -1, win_kind = -1, twT_n = -1;
@@ -761,7 +827,10 @@ template <typename R> struct Engine {
             const int ntr = b * nact;
             // stage all active atoms of b frames.  The kernel walks atoms of
             // the frame in the active-species order via the position offset.
+            int ev = tm.begin(ST_STAGE, c.st);
             stage_batch(d_pos + (size_t)f0 * n_tot * 4, b);
+            tm.end(ev, c.st);
+            ev = tm.begin(ST_SORT, c.st);
             const int nch = nchunks_upto[b];
             launch_sort_count(key.as<int>(), chunks_d.as<SortChunk>(), nch, c.g.nbins, H.as<int>(), c.st);
             launch_scan(H.as<int>(), H_upto[b], bsum.as<int64_t>(), Hs.as<int64_t>(), c.st);
@@ -770,18 +839,23 @@ template <typename R> struct Engine {
                                    cs.as<Cx<R>>(), nullptr, c.st);
             launch_binstart(Hs.as<int64_t>(), tinfo_d.as<TransformInfo>(), ntr, c.g.nbins, binstart.as<int64_t>(),
                             c.st);
+            tm.end(ev, c.st);
+            ev = tm.begin(ST_SPREAD, c.st);
             launch_spread_xfft<R>(c.g, c.rp[0], ntr, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), 0, binstart.as<int64_t>(), 1,
                                   c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), G1.as<Cx<R>>(), 0, c.st);
+            tm.end(ev, c.st);
             if (combine)
                 c.passes(ntr, b, nact, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * NQ, wsc.as<R>(),
-                         wtab.p ? wtab.as<float>() : nullptr);
+                         wtab.p ? wtab.as<float>() : nullptr, &tm);
             else
-                c.passes(ntr, ntr, 1, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * nact * NQ, nullptr, nullptr);
+                c.passes(ntr, ntr, 1, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * nact * NQ, nullptr, nullptr,
+                         &tm);
         }
         HIPCHK(hipGetLastError());
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
+        tm.collect();
         if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
     }
 
@@ -861,12 +935,17 @@ template <typename R> struct Engine {
             }
             int cols = 1;
             while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= 64) cols *= 2;
+            const int ev = tm.begin(ST_REDUCE, c.st);
             launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, c.st);
+            tm.end(ev, c.st);
         } else {
+            const int ev = tm.begin(ST_REDUCE, c.st);
             launch_sqw_sliding<R>(p, amp, r->d_qfac, d_S, d_Stot, c.st);
+            tm.end(ev, c.st);
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c.st));
+        tm.collect();
     }
 };
 
@@ -1168,6 +1247,26 @@ int mpsq_sqw_sync(mpsq_sqw e)
     if (!e) return MPSQ_ERR_BAD_ARG;
     hipStream_t st = e->ed ? e->ed->core.st : e->ef->core.st;
     return hipStreamSynchronize(st) == hipSuccess ? 0 : MPSQ_ERR_DEVICE;
+}
+
+int mpsq_sqw_set_timing(mpsq_sqw e, int on)
+{
+    if (!e) return MPSQ_ERR_BAD_ARG;
+    StageTimer &tm = e->ed ? e->ed->tm : e->ef->tm;
+    tm.on = on != 0;
+    tm.reset();
+    return 0;
+}
+
+int mpsq_sqw_get_timing(mpsq_sqw e, double *ms, int64_t *launches, int n)
+{
+    if (!e || n < 0) return MPSQ_ERR_BAD_ARG;
+    const StageTimer &tm = e->ed ? e->ed->tm : e->ef->tm;
+    for (int k = 0; k < n && k < StageTimer::K; ++k) {
+        if (ms) ms[k] = tm.ms[k];
+        if (launches) launches[k] = tm.n[k];
+    }
+    return 0;
 }
 
 int mpsq_sqw_destroy(mpsq_sqw e)

@@ -151,6 +151,17 @@ class SqwEngine:
         _lib.check("sqw_reduce", rc)
         return out, out_tot
 
+    def set_timing(self, on=True):
+        """Record HIP events around every launch group (per-stage times)."""
+        _lib.check("sqw_set_timing", self._L.mpsq_sqw_set_timing(self._h, int(bool(on))))
+
+    def timing(self):
+        """{stage: (summed ms, launches)} since set_timing(True)."""
+        ms = (C.c_double * 6)()
+        n = (C.c_int64 * 6)()
+        _lib.check("sqw_get_timing", self._L.mpsq_sqw_get_timing(self._h, ms, n, 6))
+        return {k: (ms[i], n[i]) for i, k in enumerate(_lib.STAGES)}
+
     def destroy(self):
         if getattr(self, "_h", None) and self._h.value:
             self._L.mpsq_sqw_destroy(self._h)
