@@ -300,7 +300,8 @@ template <typename R> struct Core {
         es_params(eps, sizeof(R) == 4, w, beta, warn);
         for (int d = 0; d < 3; ++d) n[d] = (d < dim) ? next_smooth_even(std::max(2 * N[d], (int64_t)2 * w)) : 1;
         const int TE = spread_tile_elems<R>();
-        if (n[0] > TE || n[1] > TE || n[2] > TE) throw Err{MPSQ_ERR_SIZE};
+        const int TC = colfft_tile_elems<R>();
+        if (n[0] > TE || n[1] > TC || n[2] > TC) throw Err{MPSQ_ERR_SIZE};
         g.dim = dim;
         g.n1 = (int)n[0];
         g.n2 = (int)n[1];
@@ -315,15 +316,17 @@ template <typename R> struct Core {
         g.s2 = (double)n[1] / (2.0 * M_PI);
         g.s3 = (double)n[2] / (2.0 * M_PI);
         g.hw = 0.5 * (double)w;
-        // spread tile: whole x extent x (by x bz) rows within the LDS tile
-        const int rows_max = (int)(TE / n[0]);
+        // spread tile: whole x extent x (by x bz) rows within the LDS tile;
+        // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
+        // one tile contiguous
+        const int rows_max = (int)std::min<int64_t>(128, TE / n[0]);
         int by = 1, bz = 1;
         if (dim == 2) {
-            for (int b = 1; b <= rows_max && n[1] % b == 0; b *= 2) by = b;
+            for (int b = 1; b <= rows_max && n[1] % b == 0 && b + w <= n[1]; b *= 2) by = b;
         } else if (dim == 3) {
             double best = 1e300;
-            for (int b1 = 1; b1 <= rows_max && n[1] % b1 == 0; b1 *= 2)
-                for (int b2 = 1; b1 * b2 <= rows_max && n[2] % b2 == 0; b2 *= 2) {
+            for (int b1 = 1; b1 <= rows_max && n[1] % b1 == 0 && b1 + w <= n[1]; b1 *= 2)
+                for (int b2 = 1; b1 * b2 <= rows_max && n[2] % b2 == 0 && b2 + w <= n[2]; b2 *= 2) {
                     const double cost = (double)(b1 + w - 1) * (b2 + w - 1) / (b1 * b2);
                     if (cost < best - 1e-12 || (std::fabs(cost - best) < 1e-12 && b1 * b2 > by * bz)) {
                         best = cost;
@@ -334,7 +337,8 @@ template <typename R> struct Core {
         }
         if (o->tile_by > 0 && dim >= 2 && n[1] % o->tile_by == 0) by = o->tile_by;
         if (o->tile_bz > 0 && dim >= 3 && n[2] % o->tile_bz == 0) bz = o->tile_bz;
-        if ((int64_t)by * bz * n[0] > TE) throw Err{MPSQ_ERR_BAD_ARG};
+        if ((int64_t)by * bz * n[0] > TE || by > 255 || bz > 255) throw Err{MPSQ_ERR_BAD_ARG};
+        if ((dim >= 2 && by + w > n[1]) || (dim >= 3 && bz + w > n[2])) throw Err{MPSQ_ERR_BAD_ARG};
         g.by = by;
         g.bz = bz;
         // sort bins: start at the tile size, coarsen until the bin count fits
@@ -377,7 +381,7 @@ template <typename R> struct Core {
 
     int colfft_cols(int64_t nlen, int64_t inner) const
     {
-        const int TE = spread_tile_elems<R>();
+        const int TE = colfft_tile_elems<R>();
         int c = 1;
         while ((int64_t)(2 * c) * nlen <= TE) c *= 2;
         while (c > 1 && c / 2 >= inner) c /= 2;
@@ -555,7 +559,8 @@ template <typename R> struct Plan {
             g1 = G1.as<Cx<R>>();
         }
         if (c.dim == 3) G2.ensure(sizeof(Cx<R>) * (size_t)(c.n[2] * c.N[1] * c.N[0]) * ntrans);
-        launch_spread_xfft<R>(c.g, c.rp[0], ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M, binstart.as<int64_t>(),
+        launch_spread_xfft<R>(c.g, c.rp[0], ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
+                              binstart.as<int64_t>(),
                               0, c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), g1,
                               (c.dim == 1 && c.iflag > 0) ? 1 : 0, c.st);
         c.passes(ntrans, ntrans, 1, g1, G2.as<Cx<R>>(), fd, nullptr, nullptr);
@@ -841,7 +846,8 @@ template <typename R> struct Engine {
                             c.st);
             tm.end(ev, c.st);
             ev = tm.begin(ST_SPREAD, c.st);
-            launch_spread_xfft<R>(c.g, c.rp[0], ntr, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), 0, binstart.as<int64_t>(), 1,
+            launch_spread_xfft<R>(c.g, c.rp[0], ntr, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), 0,
+                                  binstart.as<int64_t>(), 1,
                                   c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), G1.as<Cx<R>>(), 0, c.st);
             tm.end(ev, c.st);
             if (combine)

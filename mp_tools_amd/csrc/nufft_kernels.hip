@@ -333,162 +333,6 @@ __global__ void k_gather_strengths(int64_t M, int ntr, const int *__restrict__ p
 }
 
 // ---------------------------------------------------------------------------
-// spreading + x FFT
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
-__device__ __forceinline__ int posmod(int a, int n)
-{
-    int r = a % n;
-    return r < 0 ? r + n : r;
-}
-
-template <typename R> __device__ __forceinline__ R es_kernel(R t, R beta, R ic)
-{
-    const R a = (R)1 - t * t * ic;
-    return (a > (R)0) ? exp(beta * (sqrt(a) - (R)1)) : (R)0;
-}
-
-template <typename R, int E, int NT>
-__global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const PtRec<R> *__restrict__ srec,
-                                                    const typename cplx<R>::T *__restrict__ cs, int64_t c_tstride,
-                                                    const int64_t *__restrict__ binstart, int bin_tstride,
-                                                    const typename cplx<R>::T *__restrict__ tw1,
-                                                    const R *__restrict__ phinv1, typename cplx<R>::T *__restrict__ G1,
-                                                    int final_conj)
-{
-    using C = typename cplx<R>::T;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    C *tile = reinterpret_cast<C *>(smem_raw);
-    __shared__ int64_t rstart[MAX_NB_BINS];
-    __shared__ int rpref[MAX_NB_BINS + 1];
-    __shared__ int nrange;
-
-    const int t = blockIdx.y;
-    const int n1 = g.n1;
-    const int by = (g.dim >= 2) ? g.by : 1;
-    const int bz = (g.dim >= 3) ? g.bz : 1;
-    const int rows = by * bz;
-    const int nty = (g.dim >= 2) ? g.n2 / by : 1;
-    const int ty = blockIdx.x % nty, tz = blockIdx.x / nty;
-    const int y0 = ty * by, z0 = tz * bz;
-
-    for (int i = threadIdx.x; i < rows * n1; i += NT) tile[i] = cmk<C>((R)0, (R)0);
-
-    if (threadIdx.x == 0) {
-        const int64_t *bs = binstart + (int64_t)t * bin_tstride * (g.nbins + 1);
-        int ylo = 0, nyb = 1, zlo = 0, nzb = 1;
-        if (g.dim >= 2) {
-            ylo = floordiv(y0 - (g.w - 1), g.BY);
-            nyb = floordiv(y0 + by - 1, g.BY) - ylo + 1;
-            if (nyb >= g.nbY) { ylo = 0; nyb = g.nbY; }
-        }
-        if (g.dim >= 3) {
-            zlo = floordiv(z0 - (g.w - 1), g.BZ);
-            nzb = floordiv(z0 + bz - 1, g.BZ) - zlo + 1;
-            if (nzb >= g.nbZ) { zlo = 0; nzb = g.nbZ; }
-        }
-        int k = 0, acc = 0;
-        for (int iz = 0; iz < nzb; ++iz)
-            for (int iy = 0; iy < nyb; ++iy) {
-                const int b = (g.dim >= 3 ? posmod(zlo + iz, g.nbZ) * g.nbY : 0) + (g.dim >= 2 ? posmod(ylo + iy, g.nbY) : 0);
-                const int64_t s0 = bs[b], s1 = bs[b + 1];
-                if (s1 > s0 && k < MAX_NB_BINS) {
-                    rstart[k] = s0;
-                    rpref[k] = acc;
-                    acc += (int)(s1 - s0);
-                    ++k;
-                }
-            }
-        rpref[k] = acc;
-        nrange = k;
-    }
-    __syncthreads();
-
-    const int nr = nrange;
-    const int total = rpref[nr];
-    const R beta = (R)g.beta, ic = (R)g.ic;
-    const int w = g.w;
-    const C *cst = cs + (int64_t)t * c_tstride;
-    for (int v = threadIdx.x; v < total; v += NT) {
-        int k = 0;
-        while (k + 1 < nr && rpref[k + 1] <= v) ++k;
-        const int64_t p = rstart[k] + (v - rpref[k]);
-        const PtRec<R> pr = srec[p];
-        const C cv = cst[p];
-        // rows of the kernel inside the tile
-        R ky[MAXW], kz[MAXW];
-        int ry[MAXW], rz[MAXW];
-        int nyr = 0, nzr = 0;
-        if (g.dim >= 2) {
-            int oy = pr.ly - y0;
-            if (oy < 0) oy += g.n2;
-            for (int m = 0; m < w; ++m) {
-                int q = oy + m;
-                if (q >= g.n2) q -= g.n2;
-                if (q < by) {
-                    ry[nyr] = q;
-                    ky[nyr] = es_kernel<R>(pr.dy + (R)m, beta, ic);
-                    ++nyr;
-                }
-            }
-        } else {
-            ry[0] = 0;
-            ky[0] = (R)1;
-            nyr = 1;
-        }
-        if (g.dim >= 3) {
-            int oz = pr.lz - z0;
-            if (oz < 0) oz += g.n3;
-            for (int m = 0; m < w; ++m) {
-                int q = oz + m;
-                if (q >= g.n3) q -= g.n3;
-                if (q < bz) {
-                    rz[nzr] = q;
-                    kz[nzr] = es_kernel<R>(pr.dz + (R)m, beta, ic);
-                    ++nzr;
-                }
-            }
-        } else {
-            rz[0] = 0;
-            kz[0] = (R)1;
-            nzr = 1;
-        }
-        if (nyr == 0 || nzr == 0) continue;
-        R kx[MAXW];
-        for (int m = 0; m < w; ++m) kx[m] = es_kernel<R>(pr.dx + (R)m, beta, ic);
-        for (int iz = 0; iz < nzr; ++iz)
-            for (int iy = 0; iy < nyr; ++iy) {
-                const R wyz = ky[iy] * kz[iz];
-                const R vr = cv.x * wyz, vi = cv.y * wyz;
-                C *row = tile + (rz[iz] * by + ry[iy]) * n1;
-                int xi = pr.lx;
-                for (int m = 0; m < w; ++m) {
-                    R *cell = reinterpret_cast<R *>(row + xi);
-                    __hip_atomic_fetch_add(cell, vr * kx[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    __hip_atomic_fetch_add(cell + 1, vi * kx[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (++xi == n1) xi = 0;
-                }
-            }
-    }
-    __syncthreads();
-
-    lds_fft<C, E>(tile, rp1, rows, 1, n1, tw1, false);
-
-    // keep k1 in [-N1/2, (N1-1)/2] (CMCL index kk = k1 + N1/2), deconvolve
-    const int N1 = g.N1;
-    const int h1 = N1 / 2;
-    for (int i = threadIdx.x; i < rows * N1; i += NT) {
-        const int row = i / N1, kk = i % N1;
-        int f = kk - h1;
-        if (f < 0) f += n1;
-        C v = cscale(tile[row * n1 + f], phinv1[kk]);
-        if (final_conj) v.y = -v.y;
-        const int yy = y0 + row % by, zz = z0 + row / by;
-        G1[(((int64_t)t * g.n3 + zz) * g.n2 + yy) * N1 + kk] = v;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // column FFT pass (y or z): transform t holds [outer][n][inner]; the block
 // transforms `cols` adjacent columns of one outer slab along n in LDS,
 // keeps N of the n frequencies, deconvolves.  FINAL: additionally sums the
@@ -520,9 +364,20 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
     for (int s = 0; s < nsp; ++s) {
         const int t = FINAL ? blockIdx.y * cp.nsp + s : blockIdx.y;
         const C *src = in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
-        for (int i = threadIdx.x; i < n * cols; i += NT) {
-            const int r = i / cols, c = i % cols;
-            tile[i] = (c < ncol) ? src[(int64_t)r * inner + c] : cmk<C>((R)0, (R)0);
+        {
+            // all E loads of a thread are issued before the first LDS store
+            C v[E];
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int i = threadIdx.x + k * NT;
+                const int r = i / cols, c = i % cols;
+                v[k] = (i < n * cols && c < ncol) ? src[(int64_t)r * inner + c] : cmk<C>((R)0, (R)0);
+            }
+#pragma unroll
+            for (int k = 0; k < E; ++k) {
+                const int i = threadIdx.x + k * NT;
+                if (i < n * cols) tile[i] = v[k];
+            }
         }
         __syncthreads();
         lds_fft<C, E>(tile, rp, cols, cols, 1, tw, true);
@@ -579,6 +434,7 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
 template <typename R> struct Tiles;
 template <> struct Tiles<float> { static constexpr int E = 32, NT = 256; };
 template <> struct Tiles<double> { static constexpr int E = 16, NT = 256; };
+
 
 template <typename R>
 void launch_fold(const Geo &g, int64_t M, const void *x, const void *y, const void *z, bool in_f32, PtRec<R> *rec,
@@ -652,27 +508,7 @@ void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::
                        cs, conj);
 }
 
-template <typename R> int spread_tile_elems() { return Tiles<R>::E * Tiles<R>::NT; }
-
-template <typename R>
-void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
-                        const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
-                        const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
-                        hipStream_t st)
-{
-    constexpr int E = Tiles<R>::E, NT = Tiles<R>::NT;
-    const int by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
-    const int ntiles = ((g.dim >= 2) ? g.n2 / by : 1) * ((g.dim >= 3) ? g.n3 / bz : 1);
-    const size_t lds = (size_t)by * bz * g.n1 * sizeof(typename cplx<R>::T);
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_spread_xfft<R, E, NT>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-        attr = true;
-    }
-    hipLaunchKernelGGL((k_spread_xfft<R, E, NT>), dim3(ntiles, ntr), dim3(NT), lds, st, g, rp1, srec, cs, c_tstride,
-                       binstart, bin_tstride, tw1, phinv1, G1, final_conj);
-}
+template <typename R> int colfft_tile_elems() { return Tiles<R>::E * Tiles<R>::NT; }
 
 template <typename R>
 void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
@@ -707,10 +543,7 @@ void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const ty
                                          const PtRec<R> *, const cplx<R>::T *, PtRec<R> *, cplx<R>::T *, int *,       \
                                          hipStream_t);                                                                \
     template void launch_gather<R>(int64_t, int, const int *, const cplx<R>::T *, cplx<R>::T *, int, hipStream_t);    \
-    template int spread_tile_elems<R>();                                                                              \
-    template void launch_spread_xfft<R>(const Geo &, const RadixPlan &, int, const PtRec<R> *, const cplx<R>::T *,   \
-                                        int64_t, const int64_t *, int, const cplx<R>::T *, const R *, cplx<R>::T *,   \
-                                        int, hipStream_t);                                                            \
+    template int colfft_tile_elems<R>();                                                                              \
     template void launch_colfft<R>(const ColPass &, const RadixPlan &, int, const cplx<R>::T *, cplx<R>::T *,        \
                                    const cplx<R>::T *, const R *, const R *, const float *, bool, hipStream_t);
 

@@ -32,9 +32,20 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     const int T = (int)p.T;
     const int64_t q0 = (int64_t)blockIdx.x * cols;
     const int ncol = (int)min((int64_t)cols, p.NQ - q0);
-    for (int i = threadIdx.x; i < T * cols; i += NT) {
-        const int t = i / cols, c = i % cols;
-        tile[i] = (c < ncol) ? amp[(int64_t)t * p.NQ + q0 + c] : cmk<C>((R)0, (R)0);
+    {
+        // all loads of a thread in flight before the first LDS store
+        C v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            const int t = i / cols, c = i % cols;
+            v[k] = (i < T * cols && c < ncol) ? amp[(int64_t)t * p.NQ + q0 + c] : cmk<C>((R)0, (R)0);
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < T * cols) tile[i] = v[k];
+        }
     }
     __syncthreads();
     {
