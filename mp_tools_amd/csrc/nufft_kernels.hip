@@ -350,6 +350,8 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *tile = reinterpret_cast<C *>(smem_raw);
     const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
+    C *tw_l = tile + n * cols;  // twiddles in LDS
+    for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = tw[i];
     const int ncb = (inner + cols - 1) / cols;
     const int cb = blockIdx.x % ncb, o = blockIdx.x / ncb;
     const int col0 = cb * cols;
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
             }
         }
         __syncthreads();
-        lds_fft<C, E>(tile, rp, cols, cols, 1, tw, true);
+        lds_fft<C, E>(tile, rp, cols, cols, 1, tw_l, true);
         if (!FINAL) {
             C *dst = out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
             for (int i = threadIdx.x; i < N * cols; i += NT) {
@@ -517,7 +519,7 @@ void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const ty
 {
     constexpr int E = Tiles<R>::E, NT = Tiles<R>::NT;
     const int ncb = (cp.inner + cp.cols - 1) / cp.cols;
-    const size_t lds = (size_t)cp.n * cp.cols * sizeof(typename cplx<R>::T);
+    const size_t lds = (size_t)cp.n * (cp.cols + 1) * sizeof(typename cplx<R>::T);
     dim3 grid(ncb * cp.outer, ngroups);
     static bool attr = false;
     if (!attr) {

@@ -10,23 +10,23 @@
 //      phi(d + m) of those rows and of the w x cells, and its strength; the
 //      next chunk's records are loaded while the current one is spread;
 //   3. flattens the staged work into (point, row) items -- every item is the
-//      same w-cell update, so the lanes of a wave stay converged -- and adds
-//      the cells with FP64 LDS atomics.  gfx950 executes FP32 LDS atomic
-//      adds lane by lane (~190 cycles per wave instruction, measured) but
-//      FP64 ones at close to the plain LDS rate (~10-20 cycles), so the
-//      accumulator tile is FP64 in both precision forms.  Rows are padded by
-//      one cell so the items of one point (same x, different rows) fall in
-//      different banks;
-//   4. rounds the tile to R in place, runs the x FFT of its rows in LDS,
-//      keeps the N1 central frequencies, divides by phi_hat and writes them:
-//      the only HBM write of the pass.
+//      same w-cell update, so the lanes of a wave stay converged; each
+//      thread takes a contiguous run of items -- and adds the cells with
+//      FP64 LDS atomics.  gfx950 executes FP32 LDS atomic adds lane by lane
+//      (~190 cycles per wave instruction, measured) but FP64 ones at close
+//      to the plain LDS rate (~10-20 cycles), so the accumulator tile is FP64
+//      in both precision forms.  Rows are padded by one cell so that items
+//      with the same x in different rows fall in different banks;
+//   4. rounds the tile to R in place, runs the x FFT of its rows in LDS
+//      (twiddles copied to LDS), keeps the N1 central frequencies, divides by
+//      phi_hat and writes them: the only HBM write of the pass.
 #include "mpsq_kernels.h"
 
 namespace mpsq {
 
 template <typename R> struct SpreadCfg;
-template <> struct SpreadCfg<float> { static constexpr int E = 16, NT = 256, CH = 128; };
-template <> struct SpreadCfg<double> { static constexpr int E = 16, NT = 256, CH = 64; };
+template <> struct SpreadCfg<float> { static constexpr int E = 8, NT = 512, CH = 128; };
+template <> struct SpreadCfg<double> { static constexpr int E = 8, NT = 512, CH = 64; };
 
 __device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ int posmod(int a, int n)
@@ -35,17 +35,25 @@ __device__ __forceinline__ int posmod(int a, int n)
     return r < 0 ? r + n : r;
 }
 
-// phi(t) = exp(beta (sqrt(1 - ic t^2) - 1)), |t| <= w/2, in precision R
-template <typename R> __device__ __forceinline__ R es_kernel(R t, R beta, R ic)
+// phi(t) = exp(beta (sqrt(1 - ic t^2) - 1)), |t| <= w/2.  FP32: hardware
+// exp2 / sqrt (relative error ~1e-6 only where phi < 1e-6, i.e. absolute
+// error < 1e-12); FP64: the accurate library functions.
+__device__ __forceinline__ float es_kernel(float t, float beta, float ic)
 {
-    const R a = (R)1 - t * t * ic;
-    return (a > (R)0) ? exp(beta * (sqrt(a) - (R)1)) : (R)0;
+    const float a = 1.0f - t * t * ic;
+    return (a > 0.0f) ? __expf(beta * (__builtin_amdgcn_sqrtf(a) - 1.0f)) : 0.0f;
+}
+__device__ __forceinline__ double es_kernel(double t, double beta, double ic)
+{
+    const double a = 1.0 - t * t * ic;
+    return (a > 0.0) ? exp(beta * (sqrt(a) - 1.0)) : 0.0;
 }
 
-// exclusive scan of one int per thread over a 256-thread block; *total gets
+// exclusive scan of one int per thread over an NT-thread block; *total gets
 // the block sum on every thread
-__device__ __forceinline__ int block_scan_256(int v, int *wsum, int *total)
+template <int NT> __device__ __forceinline__ int block_scan(int v, int *wsum, int *total)
 {
+    constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int x = v;
 #pragma unroll
@@ -55,20 +63,29 @@ __device__ __forceinline__ int block_scan_256(int v, int *wsum, int *total)
     }
     if (lane == 63) wsum[wv] = x;
     __syncthreads();
-    int base = 0;
-    for (int k = 0; k < wv; ++k) base += wsum[k];
-    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    int base = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        if (k < wv) base += wsum[k];
+        tot += wsum[k];
+    }
+    *total = tot;
     __syncthreads();
     return base + x - v;
 }
 
+template <typename R> size_t spread_stage_bytes(int n1, int w, int by, int bz, int CH)
+{
+    size_t b = (size_t)CH * sizeof(typename cplx<R>::T);   // strengths
+    b += (size_t)CH * (w + by + bz) * sizeof(R);           // kernel values
+    b += (size_t)CH * 8 + (size_t)(CH + 1) * 4;            // lx, meta, item prefix
+    const size_t tw = (size_t)n1 * sizeof(typename cplx<R>::T);  // reused for the FFT twiddles
+    return ((b > tw ? b : tw) + 15) / 16 * 16;
+}
+
 template <typename R> size_t spread_lds_bytes(int rows, int n1, int w, int by, int bz, int CH)
 {
-    size_t b = (size_t)rows * (n1 + 1) * 16;           // double2 accumulators, padded rows
-    b += (size_t)CH * sizeof(typename cplx<R>::T);     // strengths
-    b += (size_t)CH * (w + by + bz) * sizeof(R);       // kernel values
-    b += (size_t)CH * 8 + (size_t)(CH + 1) * 4;        // lx, meta, item prefix
-    return (b + 15) / 16 * 16;
+    return (size_t)rows * (n1 + 1) * 16 + spread_stage_bytes<R>(n1, w, by, bz, CH);
 }
 
 template <typename R, int E, int NT, int CH>
@@ -79,12 +96,12 @@ __global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const 
                                                     const R *__restrict__ phinv1, typename cplx<R>::T *__restrict__ G1,
                                                     int final_conj)
 {
-    static_assert(NT == 256 && MAX_NB_BINS <= NT, "block scan assumes 256 threads");
+    static_assert(MAX_NB_BINS <= NT, "one thread per visited bin");
     using C = typename cplx<R>::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int64_t rstart[MAX_NB_BINS];
     __shared__ int rpref[MAX_NB_BINS + 1];
-    __shared__ int wsum[4];
+    __shared__ int wsum[NT / 64];
 
     const int t = blockIdx.y;
     const int n1 = g.n1, w = g.w, RS = g.n1 + 1;
@@ -97,7 +114,8 @@ __global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const 
     const R beta = (R)g.beta, ic = (R)g.ic;
 
     double2 *acc = reinterpret_cast<double2 *>(smem_raw);
-    unsigned char *sp = smem_raw + (size_t)rows * RS * 16;
+    unsigned char *stage = smem_raw + (size_t)rows * RS * 16;
+    unsigned char *sp = stage;
     C *st_c = reinterpret_cast<C *>(sp);
     sp += CH * sizeof(C);
     R *st_kx = reinterpret_cast<R *>(sp);
@@ -144,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const 
             cntb = (int)(bs[b + 1] - s0);
         }
         int totb;
-        const int pre = block_scan_256(cntb, wsum, &totb);
+        const int pre = block_scan<NT>(cntb, wsum, &totb);
         if ((int)threadIdx.x < nb) {
             rstart[threadIdx.x] = s0;
             rpref[threadIdx.x] = pre;
@@ -221,51 +239,59 @@ __global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const 
                 st_lx[s] = pr.lx;
                 st_meta[s] = fy | (ny << 8) | (fz << 16) | (nz << 24);
                 st_c[s] = cv;
-                for (int m = 0; m < w; ++m) st_kx[s * w + m] = es_kernel<R>(pr.dx + (R)m, beta, ic);
+                for (int m = 0; m < w; ++m) st_kx[s * w + m] = es_kernel(pr.dx + (R)m, beta, ic);
                 for (int i = 0; i < ny; ++i)
-                    st_ky[s * by + i] = (g.dim >= 2) ? es_kernel<R>(pr.dy + (R)(my + i), beta, ic) : (R)1;
+                    st_ky[s * by + i] = (g.dim >= 2) ? es_kernel(pr.dy + (R)(my + i), beta, ic) : (R)1;
                 for (int i = 0; i < nz; ++i)
-                    st_kz[s * bz + i] = (g.dim >= 3) ? es_kernel<R>(pr.dz + (R)(mz + i), beta, ic) : (R)1;
+                    st_kz[s * bz + i] = (g.dim >= 3) ? es_kernel(pr.dz + (R)(mz + i), beta, ic) : (R)1;
             }
         }
         // the next chunk's records are in flight while this one is spread
         fetch(base + CH);
         int nitems;
-        const int pre = block_scan_256(cnt, wsum, &nitems);
+        const int pre = block_scan<NT>(cnt, wsum, &nitems);
         if ((int)threadIdx.x < CH) st_pref[threadIdx.x] = pre;
         if (threadIdx.x == 0) st_pref[CH] = nitems;
         __syncthreads();
-        // ---- 3. (point, row) items, w cells each
-        for (int it = threadIdx.x; it < nitems; it += NT) {
+        // ---- 3. (point, row) items: thread k owns items [k*q, (k+1)*q)
+        const int q = (nitems + NT - 1) / NT;
+        int it = threadIdx.x * q;
+        const int it_end = min(it + q, nitems);
+        if (it < it_end) {
             int lo = 0, hi = CH;
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (st_pref[mid] <= it) lo = mid;
                 else hi = mid;
             }
-            const int s = lo;
-            const int r = it - st_pref[s];
-            const int meta = st_meta[s];
-            const int fy = meta & 255, ny = (meta >> 8) & 255, fz = (meta >> 16) & 255;
-            const int iy = r % ny, iz = r / ny;
-            const R wyz = st_ky[s * by + iy] * st_kz[s * bz + iz];
-            const C c = st_c[s];
-            const double vr = (double)(c.x * wyz), vi = (double)(c.y * wyz);
-            double2 *row = acc + ((fz + iz) * by + fy + iy) * RS;
-            int x = st_lx[s];
-            const R *kx = st_kx + s * w;
-            for (int m = 0; m < w; ++m) {
-                const double km = (double)kx[m];
-                double *cell = reinterpret_cast<double *>(row + x);
-                __hip_atomic_fetch_add(cell, vr * km, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(cell + 1, vi * km, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                if (++x == n1) x = 0;
+            int s = lo;
+            for (; it < it_end; ++it) {
+                while (st_pref[s + 1] <= it) ++s;
+                const int r = it - st_pref[s];
+                const int meta = st_meta[s];
+                const int fy = meta & 255, ny = (meta >> 8) & 255, fz = (meta >> 16) & 255;
+                const int iy = r % ny, iz = r / ny;
+                const R wyz = st_ky[s * by + iy] * st_kz[s * bz + iz];
+                const C c = st_c[s];
+                const double vr = (double)(c.x * wyz), vi = (double)(c.y * wyz);
+                double2 *row = acc + ((fz + iz) * by + fy + iy) * RS;
+                int x = st_lx[s];
+                const R *kx = st_kx + s * w;
+                for (int m = 0; m < w; ++m) {
+                    const double km = (double)kx[m];
+                    double *cell = reinterpret_cast<double *>(row + x);
+                    __hip_atomic_fetch_add(cell, vr * km, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __hip_atomic_fetch_add(cell + 1, vi * km, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (++x == n1) x = 0;
+                }
             }
         }
         __syncthreads();
     }
 
-    // ---- 4. round to R in place, x FFT, truncate + deconvolve, write
+    // ---- 4. twiddles to LDS, round to R in place, x FFT, truncate + deconvolve
+    C *tw = reinterpret_cast<C *>(stage);
+    for (int i = threadIdx.x; i < n1; i += NT) tw[i] = tw1[i];
     C *tile;
     if (sizeof(R) == 8) {
         tile = reinterpret_cast<C *>(acc);
@@ -284,9 +310,9 @@ __global__ __launch_bounds__(NT) void k_spread_xfft(Geo g, RadixPlan rp1, const 
             const int i = threadIdx.x + k * NT;
             if (i < rows * RS) tile[i] = cmk<C>((R)tmp[k].x, (R)tmp[k].y);
         }
-        __syncthreads();
     }
-    lds_fft<C, E>(tile, rp1, rows, 1, RS, tw1, false);
+    __syncthreads();
+    lds_fft<C, E>(tile, rp1, rows, 1, RS, tw, false);
 
     const int N1 = g.N1;
     const int h1 = N1 / 2;

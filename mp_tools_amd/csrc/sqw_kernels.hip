@@ -27,6 +27,8 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     using C = typename cplx<R>::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *tile = reinterpret_cast<C *>(smem_raw);
+    C *tw_l = tile + (int)p.T * cols;  // twiddles in LDS
+    for (int i = threadIdx.x; i < (int)p.T; i += NT) tw_l[i] = tw[i];
     __shared__ double red[3 * NT];
     __shared__ double mean_r[64], mean_i[64];
     const int T = (int)p.T;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         tile[i] = cmk<C>((R)(((double)v.x - mean_r[c]) * f), (R)(((double)v.y - mean_i[c]) * f));
     }
     __syncthreads();
-    lds_fft<C, E>(tile, rp, cols, cols, 1, tw, true);
+    lds_fft<C, E>(tile, rp, cols, cols, 1, tw_l, true);
     for (int i = threadIdx.x; i < p.n_f * cols; i += NT) {
         const int mi = i / cols, c = i % cols;
         if (c >= ncol) continue;
@@ -209,7 +211,7 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 {
     constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
     const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
-    const size_t lds = (size_t)p.T * cols * sizeof(typename cplx<R>::T);
+    const size_t lds = (size_t)p.T * (cols + 1) * sizeof(typename cplx<R>::T);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqw_single<R, E, NT>),
