@@ -903,7 +903,7 @@ template <typename R> struct Engine {
     void reduce(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot)
     {
         Core<R> &c = core;
-        if (!combine || !d_amp || !r || !d_S || T < 1) throw Err{MPSQ_ERR_BAD_ARG};
+        if (!combine || !d_amp || !r || (!d_S && r->n_f > 0) || T < 1) throw Err{MPSQ_ERR_BAD_ARG};
         if (r->n_int < 1 || r->n_int > T || r->n_f < 0 || !(r->nat > 0)) throw Err{MPSQ_ERR_BAD_ARG};
         HIPCHK(hipSetDevice(c.device));
         const int64_t NQ = c.N[0] * c.N[1] * c.N[2];

@@ -131,6 +131,30 @@ int orc_dirft_type1_sampled(int dim, int64_t M, const double *x,
     for (int64_t i = 0; i < 2 * K; ++i) f[i] = 0.0;
     if (K == 0 || M == 0) return 0;
     const int64_t span[3] = {hi[0] - lo[0] + 1, hi[1] - lo[1] + 1, hi[2] - lo[2] + 1};
+    if (4 * K < span[0] + span[1] + span[2]) {
+        /* few modes: one exact sincos per (point, mode) beats the tables */
+#pragma omp parallel
+        {
+            double *acc = (double *)calloc((size_t)(2 * K), sizeof(double));
+#pragma omp for schedule(static)
+            for (int64_t j = 0; j < M; ++j) {
+                const double xv = x[j], yv = dim >= 2 ? y[j] : 0.0, zv = dim >= 3 ? z[j] : 0.0;
+                const double cr = c[2 * j], ci = c[2 * j + 1];
+                for (int64_t m = 0; m < K; ++m) {
+                    double ph = (double)k1[m] * xv;
+                    if (dim >= 2) ph += (double)k2[m] * yv;
+                    if (dim >= 3) ph += (double)k3[m] * zv;
+                    const double er = cos(s * ph), ei = sin(s * ph);
+                    acc[2 * m] += cr * er - ci * ei;
+                    acc[2 * m + 1] += cr * ei + ci * er;
+                }
+            }
+#pragma omp critical
+            for (int64_t i = 0; i < 2 * K; ++i) f[i] += acc[i];
+            free(acc);
+        }
+        return 0;
+    }
     const int64_t B = 256;
     const int64_t nblk = (M + B - 1) / B;
 #pragma omp parallel
