@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=4)
     ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (5 launches) and exit: "
+                         "the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -63,6 +66,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.copy_mib:
+        copy_calibration(a.copy_mib)
+        return
     from mp_tools_amd import synth
     from mp_tools_amd.sqw import SqwEngine
 
@@ -204,6 +210,24 @@ def main():
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_leg(a, pos, counts, w, N, out32["amp"], geo)
     print(json.dumps(res))
+
+
+def copy_calibration(mib):
+    """Five launches of an elementwise kernel that reads mib MiB and writes
+    mib MiB (torch.neg on float32): the known-size kernel the HBM counters are
+    scaled by."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t) / 5
+    print(json.dumps({"copy_mib": mib, "bytes_per_launch": 2 * n * 4, "ms_per_launch": el * 1e3,
+                      "GBps": 2 * n * 4 / el / 1e9}))
 
 
 def cpu_leg(a, pos, counts, w, N, amp_gpu, geo):
