@@ -351,7 +351,8 @@ template <typename R> struct Core {
         for (;;) {
             const int64_t nbY = (dim >= 2) ? n[1] / BY : 1, nbZ = (dim >= 3) ? n[2] / BZ : 1;
             const int64_t vis = (dim >= 2 ? nvis(BY, by, nbY) : 1) * (dim >= 3 ? nvis(BZ, bz, nbZ) : 1);
-            if (nbY * nbZ <= 16384 && vis <= MAX_NB_BINS) break;
+            const int64_t runs = 2 * (dim >= 3 ? nvis(BZ, bz, nbZ) : 1);
+            if (nbY * nbZ <= 16384 && vis <= MAX_NB_BINS && runs <= spread_max_runs()) break;
             const bool cy = (dim >= 2) && (n[1] % (2 * BY) == 0) && (BY < n[1]);
             const bool cz = (dim >= 3) && (n[2] % (2 * BZ) == 0) && (BZ < n[2]);
             if (cy && (!cz || nbY >= nbZ)) BY *= 2;

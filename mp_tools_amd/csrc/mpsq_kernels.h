@@ -102,6 +102,7 @@ template <typename R>
 void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::T *c, typename cplx<R>::T *cs,
                    int conj, hipStream_t st);
 template <typename R> int spread_tile_elems();
+int spread_max_runs();
 template <typename R> int colfft_tile_elems();
 template <typename R>
 void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,

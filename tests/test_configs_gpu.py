@@ -11,7 +11,14 @@ C3: 128^3 perovskite (10,485,760 atoms), 512x512x64 modes, eps in
     CPU oracle to seconds; bench-scale runs use more).
 
 Tolerances (relative l2 over the sampled modes): FP64 form 10 * eps;
-FP32 form max(10 * eps, 5e-5).
+FP32 form max(10 * eps, 5e-5).  C1 is a nearly perfect lattice: its
+amplitudes away from the Bragg points are ~1e-3 of sum |c|, so the relative
+l2 error of the method itself (FINUFFT's, whose target is on the absolute
+error, |f~ - f| ~ eps * sum_j |c_j|) is ~2.6e-5 at eps = 1e-6 there.  C1
+is therefore checked against that bound, against the host restatement of
+the same method at the same eps (the transform the reference's finufft2d1
+call computes) to 1e-9, and at eps = 1e-9 also to 1e-7 relative l2 against
+the direct DFT.
 """
 import numpy as np
 import pytest
@@ -45,14 +52,15 @@ def _pick(amp, N, k1, k2, k3):
     return a[k3 + N[2] // 2, k2 + N[1] // 2, k1 + N[0] // 2]
 
 
-def test_c1_static_elastic_map(gpu_lib):
+@pytest.mark.parametrize("eps", [1e-6, 1e-9])
+def test_c1_static_elastic_map(gpu_lib, eps):
     import torch
     from mp_tools_amd import synth
     from mp_tools_amd.sqw import SqwEngine
     ncell, N = 20, (128, 128, 16)
     pos, counts = synth.static_frame(ncell, sigma=0.05, seed=20261203)
     w = 0.1 * synth.B_NEUTRON_FM
-    eng = SqwEngine(3, N, counts, eps=1e-6, precision="fp64", weights=w,
+    eng = SqwEngine(3, N, counts, eps=eps, precision="fp64", weights=w,
                     tpe=(2 * np.pi / ncell,) * 3)
     amp = eng.amplitudes(torch.tensor(pos, device="cuda"))
     nat = float(counts.sum())
@@ -63,9 +71,19 @@ def test_c1_static_elastic_map(gpu_lib):
     x, y, z, c = _staged(pos, counts, ncell, w)
     ref = O.dirft_type1_sampled(x, y, z, c, k1, k2, k3)
     got = _pick(amp.cpu().numpy()[0], N, k1, k2, k3)
-    assert rel_l2(got, ref) < 1e-5
+    # the method's absolute error against the direct DFT: eps is FINUFFT's
+    # target for max|f~ - f| / sum|c|, reached within 1.1-1.8x on this input
+    assert np.abs(got - ref).max() <= 3 * eps * np.abs(c).sum()
+    # the same method on the host at the same eps
+    meth = _pick(O.cpu_nufft_type1(x, y, z, c, N, eps=eps), N, k1, k2, k3)
+    err_m = rel_l2(got, meth)
+    err_d = rel_l2(got, ref)
+    print(f"C1 eps={eps:g}: rel l2 vs direct DFT {err_d:.3e}, vs host method {err_m:.3e}")
+    assert err_m < 1e-9
+    assert err_d < (5e-5 if eps >= 1e-6 else 1e-7)
     st = _pick(St.cpu().numpy(), N, k1, k2, k3)
-    assert rel_l2(st, np.abs(ref) ** 2 / nat ** 2) < 2e-5
+    assert rel_l2(st, np.abs(meth) ** 2 / nat ** 2) < 2e-9
+    assert rel_l2(st, np.abs(ref) ** 2 / nat ** 2) < (1e-4 if eps >= 1e-6 else 2e-7)
 
 
 @pytest.mark.parametrize("precision,eps", [("fp64", 1e-9), ("fp64", 1e-6), ("fp64", 1e-4), ("fp64", 1e-3),
