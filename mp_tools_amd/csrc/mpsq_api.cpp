@@ -117,7 +117,15 @@ static bool radix_plan(int n, RadixPlan &rp)
         rs.push_back(5);
     }
     if (m != 1 || rs.size() > 24) return false;
-    for (size_t i = 0; i < rs.size(); ++i) rp.r[rp.nst++] = rs[i];
+    int Ns = 1;
+    for (size_t i = 0; i < rs.size(); ++i) {
+        rp.r[rp.nst] = rs[i];
+        rp.dnb[rp.nst] = make_fdiv((uint32_t)(n / rs[i]));
+        rp.dns[rp.nst] = make_fdiv((uint32_t)Ns);
+        rp.tws[rp.nst] = n / (Ns * rs[i]);
+        Ns *= rs[i];
+        ++rp.nst;
+    }
     return true;
 }
 
@@ -300,7 +308,7 @@ template <typename R> struct Core {
         es_params(eps, sizeof(R) == 4, w, beta, warn);
         for (int d = 0; d < 3; ++d) n[d] = (d < dim) ? next_smooth_even(std::max(2 * N[d], (int64_t)2 * w)) : 1;
         const int TE = spread_tile_elems<R>();
-        const int TC = colfft_tile_elems<R>();
+        const int TC = colfft_tile_elems<R>(true);
         if (n[0] > TE || n[1] > TC || n[2] > TC) throw Err{MPSQ_ERR_SIZE};
         g.dim = dim;
         g.n1 = (int)n[0];
@@ -320,13 +328,14 @@ template <typename R> struct Core {
         // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
         // one tile contiguous
         const int rows_max = (int)std::min<int64_t>(128, TE / n[0]);
+        const int bmax = spread_max_tile_b();
         int by = 1, bz = 1;
         if (dim == 2) {
-            for (int b = 1; b <= rows_max && n[1] % b == 0 && b + w <= n[1]; b *= 2) by = b;
+            for (int b = 1; b <= rows_max && b <= bmax && n[1] % b == 0 && b + w <= n[1]; b *= 2) by = b;
         } else if (dim == 3) {
             double best = 1e300;
-            for (int b1 = 1; b1 <= rows_max && n[1] % b1 == 0 && b1 + w <= n[1]; b1 *= 2)
-                for (int b2 = 1; b1 * b2 <= rows_max && n[2] % b2 == 0 && b2 + w <= n[2]; b2 *= 2) {
+            for (int b1 = 1; b1 <= rows_max && b1 <= bmax && n[1] % b1 == 0 && b1 + w <= n[1]; b1 *= 2)
+                for (int b2 = 1; b1 * b2 <= rows_max && b2 <= bmax && n[2] % b2 == 0 && b2 + w <= n[2]; b2 *= 2) {
                     const double cost = (double)(b1 + w - 1) * (b2 + w - 1) / (b1 * b2);
                     if (cost < best - 1e-12 || (std::fabs(cost - best) < 1e-12 && b1 * b2 > by * bz)) {
                         best = cost;
@@ -337,7 +346,8 @@ template <typename R> struct Core {
         }
         if (o->tile_by > 0 && dim >= 2 && n[1] % o->tile_by == 0) by = o->tile_by;
         if (o->tile_bz > 0 && dim >= 3 && n[2] % o->tile_bz == 0) bz = o->tile_bz;
-        if ((int64_t)by * bz * n[0] > TE || by > 255 || bz > 255) throw Err{MPSQ_ERR_BAD_ARG};
+        if ((int64_t)by * bz * n[0] > TE || by > bmax || bz > bmax) throw Err{MPSQ_ERR_BAD_ARG};
+        if (spread_lds<R>(by * bz, (int)n[0], w) > 156 * 1024) throw Err{MPSQ_ERR_SIZE};
         if ((dim >= 2 && by + w > n[1]) || (dim >= 3 && bz + w > n[2])) throw Err{MPSQ_ERR_BAD_ARG};
         g.by = by;
         g.bz = bz;
@@ -380,9 +390,9 @@ template <typename R> struct Core {
 
     int keybits() const { return g.nbins > 1 ? ilog2_ceil(g.nbins) : 0; }
 
-    int colfft_cols(int64_t nlen, int64_t inner) const
+    int colfft_cols(int64_t nlen, int64_t inner, bool final) const
     {
-        const int TE = colfft_tile_elems<R>();
+        const int TE = colfft_tile_elems<R>(final);
         int c = 1;
         while ((int64_t)(2 * c) * nlen <= TE) c *= 2;
         while (c > 1 && c / 2 >= inner) c /= 2;
@@ -404,7 +414,8 @@ template <typename R> struct Core {
             cp.N = (int)N[1];
             cp.inner = (int)N[0];
             cp.outer = 1;
-            cp.cols = colfft_cols(n[1], N[0]);
+            cp.cols = colfft_cols(n[1], N[0], true);
+            cp.dcols = make_fdiv((uint32_t)cp.cols);
             cp.in_tstride = n[1] * N[0];
             cp.out_tstride = N[1] * N[0];
             cp.nsp = nsp;
@@ -419,7 +430,8 @@ template <typename R> struct Core {
             cy.N = (int)N[1];
             cy.inner = (int)N[0];
             cy.outer = (int)n[2];
-            cy.cols = colfft_cols(n[1], N[0]);
+            cy.cols = colfft_cols(n[1], N[0], false);
+            cy.dcols = make_fdiv((uint32_t)cy.cols);
             cy.in_tstride = n[2] * n[1] * N[0];
             cy.out_tstride = n[2] * N[1] * N[0];
             cy.nsp = 1;
@@ -434,7 +446,8 @@ template <typename R> struct Core {
             cz.N = (int)N[2];
             cz.inner = (int)(N[0] * N[1]);
             cz.outer = 1;
-            cz.cols = colfft_cols(n[2], N[0] * N[1]);
+            cz.cols = colfft_cols(n[2], N[0] * N[1], true);
+            cz.dcols = make_fdiv((uint32_t)cz.cols);
             cz.in_tstride = n[2] * N[1] * N[0];
             cz.out_tstride = N[2] * N[1] * N[0];
             cz.nsp = nsp;
@@ -461,7 +474,7 @@ template <typename R> struct Plan {
     int ntrans = 1;
     int64_t M = -1;
     DevBuf xyz[3], rec, key, srec, perm, H, Hs, bsum, chunks_d, tinfo_d, binstart, bad;
-    DevBuf cin, cs, G1, G2, fout;
+    DevBuf cin, cs, G1, G2, fout, amax;
     std::vector<SortChunk> chunks;
     std::vector<TransformInfo> tinfo;
 
@@ -560,10 +573,16 @@ template <typename R> struct Plan {
             g1 = G1.as<Cx<R>>();
         }
         if (c.dim == 3) G2.ensure(sizeof(Cx<R>) * (size_t)(c.n[2] * c.N[1] * c.N[0]) * ntrans);
+        const float *am = nullptr;
+        if (sizeof(R) == 4) {
+            amax.ensure(sizeof(float) * (size_t)ntrans);
+            launch_absmax(M, ntrans, reinterpret_cast<const float2 *>(cs.p), amax.as<float>(), c.st);
+            am = amax.as<float>();
+        }
         launch_spread_xfft<R>(c.g, c.rp[0], ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
                               binstart.as<int64_t>(),
                               0, c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), g1,
-                              (c.dim == 1 && c.iflag > 0) ? 1 : 0, c.st);
+                              (c.dim == 1 && c.iflag > 0) ? 1 : 0, am, c.st);
         c.passes(ntrans, ntrans, 1, g1, G2.as<Cx<R>>(), fd, nullptr, nullptr);
         HIPCHK(hipGetLastError());
         if (!on_device) {
@@ -849,7 +868,8 @@ template <typename R> struct Engine {
             ev = tm.begin(ST_SPREAD, c.st);
             launch_spread_xfft<R>(c.g, c.rp[0], ntr, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), 0,
                                   binstart.as<int64_t>(), 1,
-                                  c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), G1.as<Cx<R>>(), 0, c.st);
+                                  c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), G1.as<Cx<R>>(), 0,
+                                  nullptr, c.st);
             tm.end(ev, c.st);
             if (combine)
                 c.passes(ntr, b, nact, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * NQ, wsc.as<R>(),

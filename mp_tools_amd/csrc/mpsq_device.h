@@ -131,21 +131,60 @@ template <typename C> struct Bfly<5, C> {
     }
 };
 
-// Radix plan of one transform length: stages applied in order.
+// Division by a runtime constant d (1 <= d < 2^31) of x < 2^31 without a
+// divide: q = (umulhi(x, m) + x) >> s with s = ceil(log2 d) and
+// m = floor(2^(32+s) / d) + 1 - 2^32 (the round-up multiplier of Granlund &
+// Montgomery, PLDI 1994; the 33-bit multiplier m + 2^32 is split so that
+// every product fits 32-bit hardware).  Exact for all x < 2^31.
+struct FDiv {
+    uint32_t m;
+    uint32_t s;
+};
+__host__ __device__ inline FDiv make_fdiv(uint32_t d)
+{
+    uint32_t s = 0;
+    while (s < 31 && (1u << s) < d) ++s;
+    const uint64_t two32s = (uint64_t)1 << (32 + s);
+    FDiv f;
+    f.m = (uint32_t)(two32s / d + 1 - ((uint64_t)1 << 32));
+    f.s = s;
+    return f;
+}
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t x, FDiv f)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    const uint32_t hi = __umulhi(x, f.m);
+#else
+    const uint32_t hi = (uint32_t)(((uint64_t)x * f.m) >> 32);
+#endif
+    return (hi + x) >> f.s;
+}
+
+// Radix plan of one transform length: stages applied in order, with the
+// divisors every stage needs (n / R and the sub-transform length Ns).
 struct RadixPlan {
     int n;
     int nst;
     int r[24];
+    FDiv dnb[24];  // n / r[s]
+    FDiv dns[24];  // Ns = r[0] * ... * r[s-1]
+    int tws[24];   // twiddle stride n / (Ns * r[s])
 };
+
+// Position of element i of a row-major transform when rows are padded with
+// one spare element per 16 (keeps the stride-R writes of the early Stockham
+// stages off a single LDS bank pair).
+template <bool PAD> __device__ __forceinline__ int padx(int i) { return PAD ? i + (i >> 4) : i; }
+__host__ __device__ inline int padded_len(int n) { return n + (n >> 4) + 1; }
 
 // One Stockham stage over an LDS tile.  NT threads, each holding at most
 // ceil(E/R) butterflies (E = elements per thread the tile was sized for).
 // col_major: consecutive threads take consecutive transforms (use when the
 // transforms are the contiguous index, es > 1); otherwise consecutive threads
-// take consecutive butterflies of one transform (es == 1).
-template <typename C, int E, int R>
-__device__ __forceinline__ void stockham_stage(C *buf, int n, int cnt, int es, int ts, int Ns,
-                                               const C *__restrict__ tw, bool col_major)
+// take consecutive butterflies of one transform (es == 1, rows may be padded).
+template <typename C, int E, int R, bool PAD>
+__device__ __forceinline__ void stockham_stage(C *buf, int n, int cnt, FDiv dcnt, FDiv dnb, FDiv dns, int es, int ts,
+                                               int Ns, const C *__restrict__ tw, bool col_major)
 {
     constexpr int TPT = (E + R - 1) / R;
     const int nb = n / R;
@@ -153,23 +192,27 @@ __device__ __forceinline__ void stockham_stage(C *buf, int n, int cnt, int es, i
     const int tid = threadIdx.x, NT = blockDim.x;
     const int tstride = n / (Ns * R);
     C v[TPT * R];
+    int tt[TPT], jj[TPT];
 #pragma unroll
     for (int i = 0; i < TPT; ++i) {
         const int task = tid + i * NT;
         if (task < total) {
             int t, j;
             if (col_major) {
-                t = task % cnt;
-                j = task / cnt;
+                j = (int)fdiv((uint32_t)task, dcnt);
+                t = task - j * cnt;
             } else {
-                j = task % nb;
-                t = task / nb;
+                t = (int)fdiv((uint32_t)task, dnb);
+                j = task - t * nb;
             }
+            tt[i] = t;
+            jj[i] = j;
             const C *src = buf + t * ts;
-            const int tk = (j % Ns) * tstride;
+            const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
+            const int tk = k * tstride;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                C a = src[(j + r * nb) * es];
+                C a = src[padx<PAD>(j + r * nb) * es];
                 if (r > 0) a = cmul(a, tw[r * tk]);
                 v[i * R + r] = a;
             }
@@ -181,38 +224,33 @@ __device__ __forceinline__ void stockham_stage(C *buf, int n, int cnt, int es, i
     for (int i = 0; i < TPT; ++i) {
         const int task = tid + i * NT;
         if (task < total) {
-            int t, j;
-            if (col_major) {
-                t = task % cnt;
-                j = task / cnt;
-            } else {
-                j = task % nb;
-                t = task / nb;
-            }
+            const int t = tt[i], j = jj[i];
             C *dst = buf + t * ts;
-            const int k = j % Ns;
+            const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
             const int base = (j - k) * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[(base + r * Ns) * es] = v[i * R + r];
+            for (int r = 0; r < R; ++r) dst[padx<PAD>(base + r * Ns) * es] = v[i * R + r];
         }
     }
     __syncthreads();
 }
 
 // Full forward FFT of the tile (natural order in, natural order out).
-template <typename C, int E>
-__device__ void lds_fft(C *buf, const RadixPlan &rp, int cnt, int es, int ts, const C *__restrict__ tw,
+// dcnt = make_fdiv(cnt) (used in col_major mode).
+template <typename C, int E, bool PAD = false>
+__device__ void lds_fft(C *buf, const RadixPlan &rp, int cnt, FDiv dcnt, int es, int ts, const C *__restrict__ tw,
                         bool col_major)
 {
     int Ns = 1;
     for (int s = 0; s < rp.nst; ++s) {
         const int R = rp.r[s];
+        const FDiv dnb = rp.dnb[s], dns = rp.dns[s];
         switch (R) {
-        case 8: stockham_stage<C, E, 8>(buf, rp.n, cnt, es, ts, Ns, tw, col_major); break;
-        case 4: stockham_stage<C, E, 4>(buf, rp.n, cnt, es, ts, Ns, tw, col_major); break;
-        case 2: stockham_stage<C, E, 2>(buf, rp.n, cnt, es, ts, Ns, tw, col_major); break;
-        case 3: stockham_stage<C, E, 3>(buf, rp.n, cnt, es, ts, Ns, tw, col_major); break;
-        case 5: stockham_stage<C, E, 5>(buf, rp.n, cnt, es, ts, Ns, tw, col_major); break;
+        case 8: stockham_stage<C, E, 8, PAD>(buf, rp.n, cnt, dcnt, dnb, dns, es, ts, Ns, tw, col_major); break;
+        case 4: stockham_stage<C, E, 4, PAD>(buf, rp.n, cnt, dcnt, dnb, dns, es, ts, Ns, tw, col_major); break;
+        case 2: stockham_stage<C, E, 2, PAD>(buf, rp.n, cnt, dcnt, dnb, dns, es, ts, Ns, tw, col_major); break;
+        case 3: stockham_stage<C, E, 3, PAD>(buf, rp.n, cnt, dcnt, dnb, dns, es, ts, Ns, tw, col_major); break;
+        case 5: stockham_stage<C, E, 5, PAD>(buf, rp.n, cnt, dcnt, dnb, dns, es, ts, Ns, tw, col_major); break;
         default: break;
         }
         Ns *= R;

@@ -63,6 +63,7 @@ struct TransformInfo {
 
 struct ColPass {
     int n, N, cols, inner, outer;
+    FDiv dcols;  // make_fdiv(cols)
     int64_t in_tstride, out_tstride;
     int nsp;
     int64_t NQ;
@@ -103,12 +104,18 @@ void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::
                    int conj, hipStream_t st);
 template <typename R> int spread_tile_elems();
 int spread_max_runs();
-template <typename R> int colfft_tile_elems();
+int spread_max_tile_b();
+template <typename R> size_t spread_lds(int rows, int n1, int w);
+template <typename R> int colfft_tile_elems(bool final);
+// amax[t] bounds |Re c|, |Im c| of transform t's strengths (FP32 form only;
+// nullptr: strengths are known to be bounded by 1)
 template <typename R>
 void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
                         const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
                         const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
-                        hipStream_t st);
+                        const float *amax, hipStream_t st);
+// amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)
+void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t st);
 template <typename R>
 void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
                    typename cplx<R>::T *out, const typename cplx<R>::T *tw, const R *phinv, const R *wsc,

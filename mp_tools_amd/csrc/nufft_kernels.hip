@@ -21,6 +21,8 @@
 //                           truncation, deconvolution; the last pass also
 //                           applies species weights and sums the species
 //                           of a frame (S(Q,w) combine, mp_sqom56.f90:1338-1356).
+#include <algorithm>
+
 #include "mpsq_device.h"
 #include "mpsq_kernels.h"
 
@@ -332,6 +334,29 @@ __global__ void k_gather_strengths(int64_t M, int ntr, const int *__restrict__ p
     cs[i] = v;
 }
 
+// amax[t] = max_k max(|Re c[t][k]|, |Im c[t][k]|): non-negative floats order
+// like their bit patterns, so an integer max is exact and order-independent
+// (a NaN strength yields a NaN bound)
+__device__ __forceinline__ float nanmax(float a, float b) { return (a != a || b != b) ? __int_as_float(0x7fc00000) : fmaxf(a, b); }
+__global__ void k_zero_u32(unsigned *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__global__ void k_absmax(int64_t M, const float2 *__restrict__ c, unsigned *__restrict__ amax)
+{
+    const int tr = blockIdx.y;
+    float m = 0.0f;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < M; k += (int64_t)gridDim.x * blockDim.x) {
+        const float2 v = c[(int64_t)tr * M + k];
+        m = nanmax(m, nanmax(fabsf(v.x), fabsf(v.y)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = nanmax(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(amax + tr, __float_as_uint(m));
+}
+
 // ---------------------------------------------------------------------------
 // column FFT pass (y or z): transform t holds [outer][n][inner]; the block
 // transforms `cols` adjacent columns of one outer slab along n in LDS,
@@ -351,29 +376,26 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
     C *tile = reinterpret_cast<C *>(smem_raw);
     const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
     C *tw_l = tile + n * cols;  // twiddles in LDS
+    C *acc = tw_l + n;          // FINAL: species sum of the kept modes, [N][cols]
     for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = tw[i];
     const int ncb = (inner + cols - 1) / cols;
-    const int cb = blockIdx.x % ncb, o = blockIdx.x / ncb;
+    const int o = blockIdx.x / ncb, cb = blockIdx.x - o * ncb;
     const int col0 = cb * cols;
     const int ncol = min(cols, inner - col0);
     const int h = N / 2;
-    constexpr int NACC = (E + 1) / 2 + 1;
-    C acc[NACC];
-#pragma unroll
-    for (int i = 0; i < NACC; ++i) acc[i] = cmk<C>((R)0, (R)0);
-
     const int nsp = FINAL ? cp.nsp : 1;
     for (int s = 0; s < nsp; ++s) {
         const int t = FINAL ? blockIdx.y * cp.nsp + s : blockIdx.y;
-        const C *src = in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
         {
-            // all E loads of a thread are issued before the first LDS store
+            // all E loads of a thread are issued before the first LDS store;
+            // tile element i = threadIdx.x + k * NT is row i / cols, column i % cols
+            const C *src = in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
             C v[E];
 #pragma unroll
             for (int k = 0; k < E; ++k) {
                 const int i = threadIdx.x + k * NT;
-                const int r = i / cols, c = i % cols;
-                v[k] = (i < n * cols && c < ncol) ? src[(int64_t)r * inner + c] : cmk<C>((R)0, (R)0);
+                const int r = (int)fdiv((uint32_t)i, cp.dcols), c = i - r * cols;
+                v[k] = (i < n * cols && c < ncol) ? src[r * inner + c] : cmk<C>((R)0, (R)0);
             }
 #pragma unroll
             for (int k = 0; k < E; ++k) {
@@ -382,50 +404,38 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
             }
         }
         __syncthreads();
-        lds_fft<C, E>(tile, rp, cols, cols, 1, tw_l, true);
+        lds_fft<C, E>(tile, rp, cols, cp.dcols, cols, 1, tw_l, true);
         if (!FINAL) {
             C *dst = out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
             for (int i = threadIdx.x; i < N * cols; i += NT) {
-                const int kk = i / cols, c = i % cols;
+                const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 if (c >= ncol) continue;
                 int f = kk - h;
                 if (f < 0) f += n;
                 dst[(int64_t)kk * inner + c] = cscale(tile[f * cols + c], phinv[kk]);
             }
         } else {
-#pragma unroll
-            for (int a = 0; a < NACC; ++a) {
-                const int i = threadIdx.x + a * NT;
-                if (i < N * cols) {
-                    const int kk = i / cols, c = i % cols;
-                    if (c < ncol) {
-                        int f = kk - h;
-                        if (f < 0) f += n;
-                        R wv = phinv[kk];
-                        if (wtab) wv *= (R)wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c];
-                        else if (wsc) wv *= wsc[s];
-                        const C v = tile[f * cols + c];
-                        acc[a].x += v.x * wv;
-                        acc[a].y += v.y * wv;
-                    }
-                }
+            for (int i = threadIdx.x; i < N * cols; i += NT) {
+                const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
+                int f = kk - h;
+                if (f < 0) f += n;
+                R wv = phinv[kk];
+                if (wtab) wv *= (c < ncol) ? (R)wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c] : (R)0;
+                else if (wsc) wv *= wsc[s];
+                const C x = cscale(tile[f * cols + c], wv);
+                acc[i] = (s == 0) ? x : cadd(acc[i], x);
             }
         }
         __syncthreads();
     }
     if (FINAL) {
         C *dst = out + (int64_t)blockIdx.y * cp.out_tstride + (int64_t)o * N * inner + col0;
-#pragma unroll
-        for (int a = 0; a < NACC; ++a) {
-            const int i = threadIdx.x + a * NT;
-            if (i < N * cols) {
-                const int kk = i / cols, c = i % cols;
-                if (c < ncol) {
-                    C v = acc[a];
-                    if (cp.conj) v.y = -v.y;
-                    dst[(int64_t)kk * inner + c] = v;
-                }
-            }
+        for (int i = threadIdx.x; i < N * cols; i += NT) {
+            const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
+            if (c >= ncol) continue;
+            C x = acc[i];
+            if (cp.conj) x.y = -x.y;
+            dst[(int64_t)kk * inner + c] = x;
         }
     }
 }
@@ -434,8 +444,12 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
 // explicit instantiations + launch helpers
 // ---------------------------------------------------------------------------
 template <typename R> struct Tiles;
-template <> struct Tiles<float> { static constexpr int E = 32, NT = 256; };
-template <> struct Tiles<double> { static constexpr int E = 16, NT = 256; };
+// y pass: 512 threads (16 columns of 512 rows in FP32: 128-byte row
+// segments); final pass: 256 threads and half the tile, so that four
+// workgroups share a CU while each walks its species one after another
+template <> struct Tiles<float> { static constexpr int E = 16; };
+template <> struct Tiles<double> { static constexpr int E = 8; };
+constexpr int colfft_nt(bool final) { return final ? 256 : 512; }
 
 
 template <typename R>
@@ -510,29 +524,42 @@ void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::
                        cs, conj);
 }
 
-template <typename R> int colfft_tile_elems() { return Tiles<R>::E * Tiles<R>::NT; }
+void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t st)
+{
+    if (ntr <= 0) return;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((ntr + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<unsigned *>(amax), ntr);
+    if (M <= 0) return;
+    const int64_t nb = std::min<int64_t>((M + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)nb, (unsigned)ntr), dim3(256), 0, st, M, c,
+                       reinterpret_cast<unsigned *>(amax));
+}
+
+template <typename R> int colfft_tile_elems(bool final) { return Tiles<R>::E * colfft_nt(final); }
 
 template <typename R>
 void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
                    typename cplx<R>::T *out, const typename cplx<R>::T *tw, const R *phinv, const R *wsc,
                    const float *wtab, bool final, hipStream_t st)
 {
-    constexpr int E = Tiles<R>::E, NT = Tiles<R>::NT;
+    constexpr int E = Tiles<R>::E, NTF = colfft_nt(true), NTY = colfft_nt(false);
     const int ncb = (cp.inner + cp.cols - 1) / cp.cols;
-    const size_t lds = (size_t)cp.n * (cp.cols + 1) * sizeof(typename cplx<R>::T);
+    const size_t lds = ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) *
+                       sizeof(typename cplx<R>::T);
     dim3 grid(ncb * cp.outer, ngroups);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NT, true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTF, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NT, false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTY, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
         attr = true;
     }
     if (final)
-        hipLaunchKernelGGL((k_colfft<R, E, NT, true>), grid, dim3(NT), lds, st, cp, rp, in, out, tw, phinv, wsc, wtab);
+        hipLaunchKernelGGL((k_colfft<R, E, NTF, true>), grid, dim3(NTF), lds, st, cp, rp, in, out, tw, phinv, wsc,
+                           wtab);
     else
-        hipLaunchKernelGGL((k_colfft<R, E, NT, false>), grid, dim3(NT), lds, st, cp, rp, in, out, tw, phinv, wsc,
+        hipLaunchKernelGGL((k_colfft<R, E, NTY, false>), grid, dim3(NTY), lds, st, cp, rp, in, out, tw, phinv, wsc,
                            wtab);
 }
 
@@ -545,7 +572,7 @@ void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const ty
                                          const PtRec<R> *, const cplx<R>::T *, PtRec<R> *, cplx<R>::T *, int *,       \
                                          hipStream_t);                                                                \
     template void launch_gather<R>(int64_t, int, const int *, const cplx<R>::T *, cplx<R>::T *, int, hipStream_t);    \
-    template int colfft_tile_elems<R>();                                                                              \
+    template int colfft_tile_elems<R>(bool);                                                                          \
     template void launch_colfft<R>(const ColPass &, const RadixPlan &, int, const cplx<R>::T *, cplx<R>::T *,        \
                                    const cplx<R>::T *, const R *, const R *, const float *, bool, hipStream_t);
 

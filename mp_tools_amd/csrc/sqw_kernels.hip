@@ -18,7 +18,7 @@
 namespace mpsq {
 
 template <typename R, int E, int NT>
-__global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, int cols,
+__global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, int cols, FDiv dcols, FDiv dT,
                                                    const typename cplx<R>::T *__restrict__ amp,
                                                    const typename cplx<R>::T *__restrict__ tw,
                                                    const double *__restrict__ win, const float *__restrict__ qfac,
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
 #pragma unroll
         for (int k = 0; k < E; ++k) {
             const int i = threadIdx.x + k * NT;
-            const int t = i / cols, c = i % cols;
+            const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
             v[k] = (i < T * cols && c < ncol) ? amp[(int64_t)t * p.NQ + q0 + c] : cmk<C>((R)0, (R)0);
         }
 #pragma unroll
@@ -78,19 +78,21 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         __syncthreads();
     }
     for (int i = threadIdx.x; i < T * cols; i += NT) {
-        const int t = i / cols, c = i % cols;
+        const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
         const C v = tile[i];
         double f = win[t];
         if (qfac && c < ncol) f *= (double)qfac[q0 + c];
         tile[i] = cmk<C>((R)(((double)v.x - mean_r[c]) * f), (R)(((double)v.y - mean_i[c]) * f));
     }
     __syncthreads();
-    lds_fft<C, E>(tile, rp, cols, cols, 1, tw_l, true);
+    lds_fft<C, E>(tile, rp, cols, dcols, cols, 1, tw_l, true);
+    int m0 = (int)(p.m_lo % T);
+    if (m0 < 0) m0 += T;
     for (int i = threadIdx.x; i < p.n_f * cols; i += NT) {
-        const int mi = i / cols, c = i % cols;
+        const int mi = (int)fdiv((uint32_t)i, dcols), c = i - mi * cols;
         if (c >= ncol) continue;
-        int m = (int)((p.m_lo + mi) % T);
-        if (m < 0) m += T;
+        const int mm = m0 + mi;
+        const int m = mm - (int)fdiv((uint32_t)mm, dT) * T;
         const C v = tile[m * cols + c];
         S[(int64_t)mi * p.NQ + q0 + c] = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * p.norm_s;
     }
@@ -218,8 +220,8 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((k_sqw_single<R, E, NT>), dim3(nb), dim3(NT), lds, st, p, rp, cols, amp, tw, win, qfac, S,
-                       Stot);
+    hipLaunchKernelGGL((k_sqw_single<R, E, NT>), dim3(nb), dim3(NT), lds, st, p, rp, cols, make_fdiv((uint32_t)cols),
+                       make_fdiv((uint32_t)p.T), amp, tw, win, qfac, S, Stot);
 }
 
 template <typename R>
