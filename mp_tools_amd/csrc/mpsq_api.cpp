@@ -262,6 +262,100 @@ struct StageTimer {
 enum { ST_STAGE = 0, ST_SORT = 1, ST_SPREAD = 2, ST_COLPASS = 3, ST_FINAL = 4, ST_REDUCE = 5 };
 
 // ---------------------------------------------------------------------------
+// Job tables: built in pinned host memory and uploaded with one copy per
+// run, ahead of the kernels that read them (stream order).  Before the host
+// rewrites the tables it waits for the previous upload, which keeps a
+// pending copy from reading half-written tables.
+// ---------------------------------------------------------------------------
+struct Arena {
+    unsigned char *h = nullptr;
+    size_t cap = 0, used = 0;
+    DevBuf d;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    void begin()
+    {
+        if (pending) HIPCHK(hipEventSynchronize(ev));
+        pending = false;
+        used = 0;
+    }
+    size_t put_raw(const void *src, size_t bytes)
+    {
+        const size_t off = (used + 255) & ~(size_t)255;
+        if (off + bytes > cap) {
+            size_t nc = std::max<size_t>(cap * 2, 1 << 16);
+            while (nc < off + bytes) nc *= 2;
+            unsigned char *nh = nullptr;
+            HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&nh), nc, hipHostMallocDefault));
+            if (h) {
+                std::memcpy(nh, h, used);
+                (void)hipHostFree(h);
+            }
+            h = nh;
+            cap = nc;
+        }
+        if (bytes) std::memcpy(h + off, src, bytes);
+        used = off + bytes;
+        return off;
+    }
+    template <typename T> size_t put(const std::vector<T> &v) { return put_raw(v.data(), sizeof(T) * v.size()); }
+    void upload(hipStream_t st)
+    {
+        if (used == 0) return;
+        d.ensure(used);
+        if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIPCHK(hipMemcpyAsync(d.p, h, used, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(ev, st));
+        pending = true;
+    }
+    template <typename T> const T *dev(size_t off) const
+    {
+        return reinterpret_cast<const T *>(static_cast<const unsigned char *>(d.p) + off);
+    }
+    ~Arena()
+    {
+        if (pending && ev) (void)hipEventSynchronize(ev);
+        if (ev) (void)hipEventDestroy(ev);
+        if (h) (void)hipHostFree(h);
+    }
+};
+
+// Cut the tile-id space [0, cost.size()) into 8 contiguous ranges of about
+// equal total cost, one per XCD.
+static std::vector<int> xcd_ranges(const std::vector<double> &job_cost, const std::vector<int> &job_tiles)
+{
+    double tot = 0.0;
+    int ntile = 0;
+    for (size_t j = 0; j < job_cost.size(); ++j) {
+        tot += job_cost[j] * job_tiles[j];
+        ntile += job_tiles[j];
+    }
+    std::vector<int> xr(9, 0);
+    xr[8] = ntile;
+    int tile = 0;
+    double acc = 0.0;
+    size_t jj = 0;
+    int in_job = 0;
+    for (int x = 1; x < 8; ++x) {
+        const double target = tot * x / 8.0;
+        while (jj < job_cost.size() && acc + job_cost[jj] * (job_tiles[jj] - in_job) <= target) {
+            acc += job_cost[jj] * (job_tiles[jj] - in_job);
+            tile += job_tiles[jj] - in_job;
+            ++jj;
+            in_job = 0;
+        }
+        if (jj < job_cost.size() && job_cost[jj] > 0) {
+            const int k = std::min(job_tiles[jj] - in_job, (int)((target - acc) / job_cost[jj]));
+            acc += job_cost[jj] * k;
+            tile += k;
+            in_job += k;
+        }
+        xr[x] = tile;
+    }
+    return xr;
+}
+
+// ---------------------------------------------------------------------------
 // geometry shared by plans and the S(Q,w) engine
 // ---------------------------------------------------------------------------
 template <typename R> struct Core {
@@ -399,64 +493,96 @@ template <typename R> struct Core {
         return c;
     }
 
-    // Launch the transform passes after spreading.  ntr transforms are laid
-    // out back to back; FINAL groups nsp consecutive transforms.
-    void passes(int ntr, int ngroups, int nsp, const Cx<R> *G1, Cx<R> *G2, Cx<R> *out, const R *wsc,
-                const float *wtab, StageTimer *tm = nullptr)
+    // Column passes after spreading.  2D: one final pass along y.  3D: a
+    // y pass (every transform) and a final z pass that groups nsp
+    // consecutive transforms (species) with their weights.
+    ColPass colpass(int axis, bool final, int nsp) const
     {
-        StageTimer dummy;
-        if (!tm) tm = &dummy;
-        const int conj = iflag > 0 ? 1 : 0;
+        ColPass c{};
         const int64_t NQ = N[0] * N[1] * N[2];
-        if (dim == 2) {
-            ColPass cp{};
-            cp.n = (int)n[1];
-            cp.N = (int)N[1];
-            cp.inner = (int)N[0];
-            cp.outer = 1;
-            cp.cols = colfft_cols(n[1], N[0], true);
-            cp.dcols = make_fdiv((uint32_t)cp.cols);
-            cp.in_tstride = n[1] * N[0];
-            cp.out_tstride = N[1] * N[0];
-            cp.nsp = nsp;
-            cp.NQ = NQ;
-            cp.conj = conj;
-            const int ev = tm->begin(ST_FINAL, st);
-            launch_colfft<R>(cp, rp[1], ngroups, G1, out, tw[1].as<Cx<R>>(), phinv[1].as<R>(), wsc, wtab, true, st);
-            tm->end(ev, st);
-        } else if (dim == 3) {
-            ColPass cy{};
-            cy.n = (int)n[1];
-            cy.N = (int)N[1];
-            cy.inner = (int)N[0];
-            cy.outer = (int)n[2];
-            cy.cols = colfft_cols(n[1], N[0], false);
-            cy.dcols = make_fdiv((uint32_t)cy.cols);
-            cy.in_tstride = n[2] * n[1] * N[0];
-            cy.out_tstride = n[2] * N[1] * N[0];
-            cy.nsp = 1;
-            cy.NQ = NQ;
-            cy.conj = 0;
-            const int ev1 = tm->begin(ST_COLPASS, st);
-            launch_colfft<R>(cy, rp[1], ntr, G1, G2, tw[1].as<Cx<R>>(), phinv[1].as<R>(), nullptr, nullptr, false,
-                             st);
-            tm->end(ev1, st);
-            ColPass cz{};
-            cz.n = (int)n[2];
-            cz.N = (int)N[2];
-            cz.inner = (int)(N[0] * N[1]);
-            cz.outer = 1;
-            cz.cols = colfft_cols(n[2], N[0] * N[1], true);
-            cz.dcols = make_fdiv((uint32_t)cz.cols);
-            cz.in_tstride = n[2] * N[1] * N[0];
-            cz.out_tstride = N[2] * N[1] * N[0];
-            cz.nsp = nsp;
-            cz.NQ = NQ;
-            cz.conj = conj;
-            const int ev2 = tm->begin(ST_FINAL, st);
-            launch_colfft<R>(cz, rp[2], ngroups, G2, out, tw[2].as<Cx<R>>(), phinv[2].as<R>(), wsc, wtab, true, st);
-            tm->end(ev2, st);
+        c.n = (int)n[axis];
+        c.N = (int)N[axis];
+        if (axis == 1) {
+            c.inner = (int)N[0];
+            c.outer = (dim == 3) ? (int)n[2] : 1;
+            c.in_tstride = n[2] * n[1] * N[0];
+            c.out_tstride = (dim == 3) ? n[2] * N[1] * N[0] : N[1] * N[0];
+        } else {
+            c.inner = (int)(N[0] * N[1]);
+            c.outer = 1;
+            c.in_tstride = n[2] * N[1] * N[0];
+            c.out_tstride = NQ;
         }
+        c.cols = colfft_cols(c.n, c.inner, final);
+        c.dcols = make_fdiv((uint32_t)c.cols);
+        c.nsp = final ? nsp : 1;
+        c.NQ = NQ;
+        c.conj = (final && iflag > 0) ? 1 : 0;
+        return c;
+    }
+    ColJob<R> coljob(int axis, bool final, int nsp, int ngroups, const Cx<R> *in, Cx<R> *out, const R *wsc,
+                     const float *wtab) const
+    {
+        ColJob<R> J{};
+        J.cp = colpass(axis, final, nsp);
+        J.rp = rp[axis];
+        J.in = in;
+        J.out = out;
+        J.tw = tw[axis].template as<Cx<R>>();
+        J.phinv = phinv[axis].template as<R>();
+        J.wsc = final ? wsc : nullptr;
+        J.wtab = final ? wtab : nullptr;
+        J.ncb = (J.cp.inner + J.cp.cols - 1) / J.cp.cols;
+        J.ngroups = ngroups;
+        J.blk0 = 0;
+        return J;
+    }
+    static int64_t col_blocks(const ColJob<R> &J) { return (int64_t)J.ncb * J.cp.outer * J.ngroups; }
+
+    int ntiles() const
+    {
+        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        return ((dim >= 2) ? g.n2 / by : 1) * ((dim >= 3) ? g.n3 / bz : 1);
+    }
+    size_t spread_lds_size() const
+    {
+        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        return spread_lds<R>(by * bz, g.n1, g.w);
+    }
+    SpreadJob<R> spreadjob(int ntr, const PtRec<R> *srec, const Cx<R> *cs, int64_t c_tstride, const int64_t *bstart,
+                           int bin_tstride, Cx<R> *G1, const float *amax, int final_conj) const
+    {
+        SpreadJob<R> J{};
+        J.g = g;
+        J.rp1 = rp[0];
+        J.srec = srec;
+        J.cs = cs;
+        J.c_tstride = c_tstride;
+        J.binstart = bstart;
+        J.bin_tstride = bin_tstride;
+        J.tw1 = tw[0].template as<Cx<R>>();
+        J.phinv1 = phinv[0].template as<R>();
+        J.G1 = G1;
+        J.amax = amax;
+        J.final_conj = final_conj;
+        J.ntr = ntr;
+        J.ntiles = ntiles();
+        J.tile0 = 0;
+        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        J.dntr = make_fdiv((uint32_t)ntr);
+        J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
+        J.dby = make_fdiv((uint32_t)by);
+        J.lds_tw = (int)(spread_lds_size() - (size_t)g.n1 * sizeof(Cx<R>));
+        return J;
+    }
+    // estimated cost of one tile of this geometry with M points per transform
+    // (candidates it visits plus a fixed part for zeroing / FFT / output)
+    double tile_cost(int64_t M) const
+    {
+        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        const double halo = (double)(by + (dim >= 2 ? g.w - 1 : 0)) * (bz + (dim >= 3 ? g.w - 1 : 0));
+        const double cand = (double)M * halo / ((double)g.n2 * g.n3);
+        return cand * g.w + (double)by * bz * g.n1 * 1.5;
     }
 
     ~Core()
@@ -473,10 +599,11 @@ template <typename R> struct Plan {
     Core<R> core;
     int ntrans = 1;
     int64_t M = -1;
-    DevBuf xyz[3], rec, key, srec, perm, H, Hs, bsum, chunks_d, tinfo_d, binstart, bad;
+    DevBuf xyz[3], rec, key, srec, perm, H, Hs, bsum, binstart, bad, wc;
     DevBuf cin, cs, G1, G2, fout, amax;
-    std::vector<SortChunk> chunks;
-    std::vector<TransformInfo> tinfo;
+    Arena sort_tab, run_tab;
+    int nch = 0;
+    size_t off_chunks = 0, off_tinfo = 0;
 
     void setpts(int64_t M_, const R *x, const R *y, const R *z, bool on_device)
     {
@@ -507,33 +634,42 @@ template <typename R> struct Plan {
         launch_fold<R>(c.g, M_, dptr[0], dptr[1], dptr[2], sizeof(R) == 4, rec.as<PtRec<R>>(), key.as<int>(),
                        bad.as<int>(), c.st);
         const int64_t CH = c.sort_chunk;
-        const int nch = (int)((M_ + CH - 1) / CH);
-        chunks.resize((size_t)std::max(nch, 1));
+        nch = (int)((M_ + CH - 1) / CH);
+        std::vector<SortChunk> chunks((size_t)std::max(nch, 1));
         for (int k = 0; k < nch; ++k) {
-            chunks[k].p0 = (int64_t)k * CH;
-            chunks[k].p1 = std::min<int64_t>(M_, (int64_t)(k + 1) * CH);
-            chunks[k].pbase = 0;
-            chunks[k].hbase = 0;
-            chunks[k].nch = nch;
-            chunks[k].lc = k;
+            SortChunk &ch = chunks[k];
+            ch.p0 = (int64_t)k * CH;
+            ch.p1 = std::min<int64_t>(M_, (int64_t)(k + 1) * CH);
+            ch.pbase = 0;
+            ch.hbase = 0;
+            ch.nch = nch;
+            ch.lc = k;
+            ch.nbins = c.g.nbins;
+            ch.keybits = c.keybits();
         }
-        tinfo.assign(1, TransformInfo{0, M_, 0, nch, 0});
+        std::vector<TransformInfo> tinfo(1);
+        tinfo[0].p0 = 0;
+        tinfo[0].p1 = M_;
+        tinfo[0].hbase = 0;
+        tinfo[0].bs0 = 0;
+        tinfo[0].nch = nch;
+        tinfo[0].nbins = c.g.nbins;
+        sort_tab.begin();
+        off_chunks = sort_tab.put(chunks);
+        off_tinfo = sort_tab.put(tinfo);
+        sort_tab.upload(c.st);
         const int64_t Htot = (int64_t)c.g.nbins * nch;
         H.ensure(sizeof(int) * (size_t)std::max<int64_t>(Htot, 1));
         Hs.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(Htot, 1));
         bsum.ensure(sizeof(int64_t) * (size_t)scan_bsum_size(Htot));
-        chunks_d.ensure(sizeof(SortChunk) * chunks.size());
-        tinfo_d.ensure(sizeof(TransformInfo));
         binstart.ensure(sizeof(int64_t) * (size_t)(c.g.nbins + 1));
-        HIPCHK(hipMemcpyAsync(chunks_d.p, chunks.data(), sizeof(SortChunk) * chunks.size(), hipMemcpyHostToDevice,
-                              c.st));
-        HIPCHK(hipMemcpyAsync(tinfo_d.p, tinfo.data(), sizeof(TransformInfo), hipMemcpyHostToDevice, c.st));
-        launch_sort_count(key.as<int>(), chunks_d.as<SortChunk>(), nch, c.g.nbins, H.as<int>(), c.st);
+        const SortChunk *chd = sort_tab.dev<SortChunk>(off_chunks);
+        launch_sort_count(key.as<int>(), chd, nch, c.g.nbins, H.as<int>(), c.st);
         launch_scan(H.as<int>(), Htot, bsum.as<int64_t>(), Hs.as<int64_t>(), c.st);
-        launch_sort_scatter<R>(key.as<int>(), chunks_d.as<SortChunk>(), nch, c.g.nbins, c.keybits(),
-                               Hs.as<int64_t>(), rec.as<PtRec<R>>(), nullptr, srec.as<PtRec<R>>(), nullptr,
-                               perm.as<int>(), c.st);
-        launch_binstart(Hs.as<int64_t>(), tinfo_d.as<TransformInfo>(), 1, c.g.nbins, binstart.as<int64_t>(), c.st);
+        launch_sort_scatter<R>(key.as<int>(), chd, nch, c.g.nbins, Hs.as<int64_t>(), rec.as<PtRec<R>>(), nullptr,
+                               srec.as<PtRec<R>>(), nullptr, perm.as<int>(), c.st);
+        launch_binstart(Hs.as<int64_t>(), sort_tab.dev<TransformInfo>(off_tinfo), 1, c.g.nbins,
+                        binstart.as<int64_t>(), c.st);
         HIPCHK(hipGetLastError());
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
@@ -579,11 +715,33 @@ template <typename R> struct Plan {
             launch_absmax(M, ntrans, reinterpret_cast<const float2 *>(cs.p), amax.as<float>(), c.st);
             am = amax.as<float>();
         }
-        launch_spread_xfft<R>(c.g, c.rp[0], ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
-                              binstart.as<int64_t>(),
-                              0, c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), g1,
-                              (c.dim == 1 && c.iflag > 0) ? 1 : 0, am, c.st);
-        c.passes(ntrans, ntrans, 1, g1, G2.as<Cx<R>>(), fd, nullptr, nullptr);
+        // one-entry job tables: spread, then the column passes
+        std::vector<SpreadJob<R>> sj(1, c.spreadjob(ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
+                                                     binstart.as<int64_t>(), 0, g1, am,
+                                                     (c.dim == 1 && c.iflag > 0) ? 1 : 0));
+        std::vector<int> zero1(1, 0);
+        const int ntile = sj[0].ntiles * ntrans;
+        std::vector<int> xr = xcd_ranges(std::vector<double>(1, 1.0), std::vector<int>(1, ntile));
+        std::vector<ColJob<R>> cy, cz;
+        if (c.dim == 2) {
+            cz.push_back(c.coljob(1, true, 1, ntrans, g1, fd, nullptr, nullptr));
+        } else if (c.dim == 3) {
+            cy.push_back(c.coljob(1, false, 1, ntrans, g1, G2.as<Cx<R>>(), nullptr, nullptr));
+            cz.push_back(c.coljob(2, true, 1, ntrans, G2.as<Cx<R>>(), fd, nullptr, nullptr));
+        }
+        run_tab.begin();
+        const size_t o_sj = run_tab.put(sj), o_z = run_tab.put(zero1), o_xr = run_tab.put(xr);
+        const size_t o_cy = run_tab.put(cy), o_cz = run_tab.put(cz);
+        run_tab.upload(c.st);
+        wc.ensure(sizeof(int) * 8);
+        launch_spread_jobs<R>(run_tab.dev<SpreadJob<R>>(o_sj), run_tab.dev<int>(o_z), 1, c.g.w,
+                              run_tab.dev<int>(o_xr), wc.as<int>(), c.spread_lds_size(), c.st);
+        if (!cy.empty())
+            launch_colfft_jobs<R>(run_tab.dev<ColJob<R>>(o_cy), run_tab.dev<int>(o_z), 1,
+                                  (int)Core<R>::col_blocks(cy[0]), false, colfft_lds<R>(cy[0].cp, false), c.st);
+        if (!cz.empty())
+            launch_colfft_jobs<R>(run_tab.dev<ColJob<R>>(o_cz), run_tab.dev<int>(o_z), 1,
+                                  (int)Core<R>::col_blocks(cz[0]), true, colfft_lds<R>(cz[0].cp, true), c.st);
         HIPCHK(hipGetLastError());
         if (!on_device) {
             HIPCHK(hipMemcpyAsync(f_out, fd, sizeof(Cx<R>) * (size_t)NQ * ntrans, hipMemcpyDeviceToHost, c.st));
@@ -687,6 +845,28 @@ template <typename R> static int info_impl(Plan<R> *p, int64_t *out)
 // ---------------------------------------------------------------------------
 constexpr int MAXSP = 64;
 
+template <typename R> struct Engine;
+
+// Device buffers of one run (any number of engine parts): points (staged,
+// sorted), the sort's count matrix, bin starts, the two intermediate grids,
+// and the job tables.
+template <typename R> struct Workspace {
+    DevBuf rec, key, cstr, srec, cs, H, Hs, bsum, binstart, G1, G2, bad, wc;
+    Arena tab;
+};
+
+// A part of a run: frames [f0, f0 + b) of one engine's trajectory.
+template <typename R> struct Part {
+    Engine<R> *e;
+    int64_t f0;
+    int b;
+    const float *pos;  // frame f0 of the trajectory [T][n_tot][4]
+    void *amp;         // output of frame f0 ([b][NQ] combined, [b][nact][NQ] per species)
+};
+
+template <typename R>
+static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipStream_t st, StageTimer *tm);
+
 template <typename R> struct Engine {
     Core<R> core;
     int precision = MPSQ_FP32;
@@ -698,12 +878,9 @@ template <typename R> struct Engine {
     int64_t Mact = 0;
     int combine = 1;
     StageParams sp{};
-    int B = 1;                      // frames per batch
-    DevBuf wsc, wtab, at_ind, rec, key, cstr, srec, cs, H, Hs, bsum, chunks_d, tinfo_d, binstart, bad, G1, G2;
-    std::vector<SortChunk> chunks;      // for B frames
-    std::vector<int> nchunks_upto;      // chunks of the first b frames
-    std::vector<int64_t> H_upto;        // count-matrix size of the first b frames
-    std::vector<TransformInfo> tinfo;
+    int B = 1;                      // frames per batch when run alone
+    DevBuf wsc, wtab, at_ind;
+    Workspace<R> ws;                // buffers when run alone
     // reduction caches
     DevBuf win_d, twT;
     StageTimer tm;
@@ -711,6 +888,17 @@ template <typename R> struct Engine {
     std::vector<Cx<R>> twT_h;
     int win_n = -1, win_kind = -1, twT_n = -1;
     RadixPlan rpT{};
+
+    int64_t nq() const { return core.N[0] * core.N[1] * core.N[2]; }
+    int64_t amp_frame_elems() const { return combine ? nq() : nq() * nact; }
+    // device bytes of intermediates per frame (points, sort keys, grids)
+    size_t bytes_per_frame() const
+    {
+        return (size_t)Mact * (2 * sizeof(PtRec<R>) + 2 * sizeof(Cx<R>) + 2 * sizeof(int)) +
+               (size_t)nact * sizeof(Cx<R>) *
+                   (size_t)(core.n[2] * core.n[1] * core.N[0] +
+                            (core.dim == 3 ? core.n[2] * core.N[1] * core.N[0] : 0));
+    }
 
     void create(const mpsq_sqw_spec *s)
     {
@@ -739,7 +927,7 @@ template <typename R> struct Engine {
         if (nact == 0) throw Err{MPSQ_ERR_BAD_ARG};
         Mact = act_off.back();
         if (Mact > INT32_MAX) throw Err{MPSQ_ERR_SIZE};
-        const int64_t NQ = core.N[0] * core.N[1] * core.N[2];
+        const int64_t NQ = nq();
         // weights of the active species
         std::vector<R> wh((size_t)nact);
         for (int k = 0; k < nact; ++k) wh[k] = s->w_scalar ? (R)s->w_scalar[act[k]] : (R)1;
@@ -773,70 +961,9 @@ template <typename R> struct Engine {
             HIPCHK(hipMemcpy(at_ind.p, s->at_ind, sizeof(int) * 4 * (size_t)n_tot, hipMemcpyHostToDevice));
         }
         // batch size: intermediate buffers of about 2 GiB per batch, <= 16 frames
-        const size_t per_frame = (size_t)Mact * (2 * sizeof(PtRec<R>) + 2 * sizeof(Cx<R>) + 2 * sizeof(int)) +
-                                 (size_t)nact * sizeof(Cx<R>) *
-                                     (size_t)(core.n[2] * core.n[1] * core.N[0] +
-                                              (core.dim == 3 ? core.n[2] * core.N[1] * core.N[0] : 0));
-        int b = s->batch_frames > 0 ? s->batch_frames : (int)std::max<size_t>(1, ((size_t)2 << 30) / per_frame);
+        int b = s->batch_frames > 0 ? s->batch_frames
+                                    : (int)std::max<size_t>(1, ((size_t)2 << 30) / bytes_per_frame());
         B = std::max(1, std::min(b, 16));
-        build_chunks();
-        alloc_batch();
-    }
-
-    // sort chunk table for B frames; transform t = b*nact + k, points of
-    // t are [b*Mact + act_off[k], b*Mact + act_off[k+1])
-    void build_chunks()
-    {
-        const int64_t CH = core.sort_chunk;
-        chunks.clear();
-        tinfo.clear();
-        nchunks_upto.assign((size_t)B + 1, 0);
-        H_upto.assign((size_t)B + 1, 0);
-        int64_t hbase = 0;
-        for (int b = 0; b < B; ++b) {
-            for (int k = 0; k < nact; ++k) {
-                const int64_t p0 = (int64_t)b * Mact + act_off[k], p1 = (int64_t)b * Mact + act_off[k + 1];
-                const int nch = (int)((p1 - p0 + CH - 1) / CH);
-                for (int c = 0; c < nch; ++c) {
-                    SortChunk ch;
-                    ch.p0 = p0 + (int64_t)c * CH;
-                    ch.p1 = std::min(p1, p0 + (int64_t)(c + 1) * CH);
-                    ch.pbase = p0;
-                    ch.hbase = hbase;
-                    ch.nch = nch;
-                    ch.lc = c;
-                    chunks.push_back(ch);
-                }
-                tinfo.push_back(TransformInfo{p0, p1, hbase, nch, 0});
-                hbase += (int64_t)core.g.nbins * nch;
-            }
-            nchunks_upto[b + 1] = (int)chunks.size();
-            H_upto[b + 1] = hbase;
-        }
-    }
-
-    void alloc_batch()
-    {
-        const size_t np = (size_t)B * (size_t)std::max<int64_t>(Mact, 1);
-        rec.ensure(sizeof(PtRec<R>) * np);
-        srec.ensure(sizeof(PtRec<R>) * np);
-        key.ensure(sizeof(int) * np);
-        cstr.ensure(sizeof(Cx<R>) * np);
-        cs.ensure(sizeof(Cx<R>) * np);
-        const int64_t Htot = std::max<int64_t>(H_upto[B], 1);
-        H.ensure(sizeof(int) * (size_t)Htot);
-        Hs.ensure(sizeof(int64_t) * (size_t)Htot);
-        bsum.ensure(sizeof(int64_t) * (size_t)scan_bsum_size(Htot));
-        chunks_d.ensure(sizeof(SortChunk) * std::max<size_t>(chunks.size(), 1));
-        tinfo_d.ensure(sizeof(TransformInfo) * std::max<size_t>(tinfo.size(), 1));
-        binstart.ensure(sizeof(int64_t) * (size_t)(core.g.nbins + 1) * tinfo.size());
-        bad.ensure(sizeof(int));
-        if (!chunks.empty())
-            HIPCHK(hipMemcpy(chunks_d.p, chunks.data(), sizeof(SortChunk) * chunks.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(tinfo_d.p, tinfo.data(), sizeof(TransformInfo) * tinfo.size(), hipMemcpyHostToDevice));
-        const size_t ntr = (size_t)B * nact;
-        G1.ensure(sizeof(Cx<R>) * ntr * (size_t)(core.n[2] * core.n[1] * core.N[0]));
-        if (core.dim == 3) G2.ensure(sizeof(Cx<R>) * ntr * (size_t)(core.n[2] * core.N[1] * core.N[0]));
     }
 
     void amplitudes(const float *d_pos, int64_t T, void *d_amp)
@@ -844,81 +971,24 @@ template <typename R> struct Engine {
         Core<R> &c = core;
         if (!d_pos || !d_amp || T < 1) throw Err{MPSQ_ERR_BAD_ARG};
         HIPCHK(hipSetDevice(c.device));
-        const int64_t NQ = c.N[0] * c.N[1] * c.N[2];
-        HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(int), c.st));
-        Cx<R> *amp = static_cast<Cx<R> *>(d_amp);
+        ws.bad.ensure(sizeof(int));
+        HIPCHK(hipMemsetAsync(ws.bad.p, 0, sizeof(int), c.st));
         for (int64_t f0 = 0; f0 < T; f0 += B) {
             const int b = (int)std::min<int64_t>(B, T - f0);
-            const int ntr = b * nact;
-            // stage all active atoms of b frames.  The kernel walks atoms of
-            // the frame in the active-species order via the position offset.
-            int ev = tm.begin(ST_STAGE, c.st);
-            stage_batch(d_pos + (size_t)f0 * n_tot * 4, b);
-            tm.end(ev, c.st);
-            ev = tm.begin(ST_SORT, c.st);
-            const int nch = nchunks_upto[b];
-            launch_sort_count(key.as<int>(), chunks_d.as<SortChunk>(), nch, c.g.nbins, H.as<int>(), c.st);
-            launch_scan(H.as<int>(), H_upto[b], bsum.as<int64_t>(), Hs.as<int64_t>(), c.st);
-            launch_sort_scatter<R>(key.as<int>(), chunks_d.as<SortChunk>(), nch, c.g.nbins, c.keybits(),
-                                   Hs.as<int64_t>(), rec.as<PtRec<R>>(), cstr.as<Cx<R>>(), srec.as<PtRec<R>>(),
-                                   cs.as<Cx<R>>(), nullptr, c.st);
-            launch_binstart(Hs.as<int64_t>(), tinfo_d.as<TransformInfo>(), ntr, c.g.nbins, binstart.as<int64_t>(),
-                            c.st);
-            tm.end(ev, c.st);
-            ev = tm.begin(ST_SPREAD, c.st);
-            launch_spread_xfft<R>(c.g, c.rp[0], ntr, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), 0,
-                                  binstart.as<int64_t>(), 1,
-                                  c.tw[0].template as<Cx<R>>(), c.phinv[0].template as<R>(), G1.as<Cx<R>>(), 0,
-                                  nullptr, c.st);
-            tm.end(ev, c.st);
-            if (combine)
-                c.passes(ntr, b, nact, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * NQ, wsc.as<R>(),
-                         wtab.p ? wtab.as<float>() : nullptr, &tm);
-            else
-                c.passes(ntr, ntr, 1, G1.as<Cx<R>>(), G2.as<Cx<R>>(), amp + (size_t)f0 * nact * NQ, nullptr, nullptr,
-                         &tm);
+            std::vector<Part<R>> parts(1);
+            parts[0].e = this;
+            parts[0].f0 = f0;
+            parts[0].b = b;
+            parts[0].pos = d_pos + (size_t)f0 * n_tot * 4;
+            parts[0].amp = static_cast<Cx<R> *>(d_amp) + (size_t)f0 * amp_frame_elems();
+            run_parts<R>(parts, ws, c.st, &tm);
         }
         HIPCHK(hipGetLastError());
         int hbad = 0;
-        HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipMemcpyAsync(&hbad, ws.bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         tm.collect();
         if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
-    }
-
-    // Stage b frames.  When every species is active the batch's points are
-    // the frames' atoms in storage order; otherwise one launch per active
-    // species block maps its atoms onto the active point range.
-    void stage_batch(const float *pos, int b)
-    {
-        Core<R> &c = core;
-        const float4 *p4 = reinterpret_cast<const float4 *>(pos);
-        const int4 *ai = at_ind.p ? at_ind.as<int4>() : nullptr;
-        if (nact == nsp) {
-            launch_stage<R>(c.g, sp, (int64_t)b * n_tot, p4, ai, rec.as<PtRec<R>>(), key.as<int>(),
-                            cstr.as<Cx<R>>(), bad.as<int>(), c.st);
-            return;
-        }
-        for (int fb = 0; fb < b; ++fb)
-            for (int k = 0; k < nact; ++k) {
-                const int j = act[k];
-                const int64_t cnt = sp_count[j];
-                if (cnt == 0) continue;
-                StageParams s2 = sp;
-                s2.n_tot = n_tot;  // atom index = p % n_tot stays valid with the offsets below
-                const int64_t dst = (int64_t)fb * Mact + act_off[k];
-                // the kernel indexes pos[p] and at_ind[p % n_tot]; pass views
-                // shifted so that p = 0 is atom sp_off[j] of frame fb
-                launch_stage_view(s2, p4 + (size_t)fb * n_tot + sp_off[j], ai ? ai + sp_off[j] : nullptr, cnt, dst);
-            }
-    }
-
-    void launch_stage_view(StageParams s2, const float4 *p4, const int4 *ai, int64_t cnt, int64_t dst)
-    {
-        Core<R> &c = core;
-        s2.n_tot = cnt;  // at_ind view indexed by p % cnt == p
-        launch_stage<R>(c.g, s2, cnt, p4, ai, rec.as<PtRec<R>>() + dst, key.as<int>() + dst, cstr.as<Cx<R>>() + dst,
-                        bad.as<int>(), c.st);
     }
 
     void reduce(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot)
@@ -975,6 +1045,221 @@ template <typename R> struct Engine {
         tm.collect();
     }
 };
+
+// ---------------------------------------------------------------------------
+// One run of engine parts: every phase is one launch over all parts.
+//   stage    points of all parts (active species, species-major per frame)
+//   sort     count / scan / stable scatter / bin starts over all transforms
+//   spread   one persistent launch over the tiles of all transforms
+//            (parts are grouped by kernel width; one launch per width)
+//   columns  y pass over all transforms, final pass (species combine)
+// ---------------------------------------------------------------------------
+template <typename R>
+static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipStream_t st, StageTimer *tm)
+{
+    StageTimer dummy;
+    if (!tm) tm = &dummy;
+    const int np = (int)parts.size();
+    if (np == 0) return;
+    // ---- layout of the parts in the shared buffers
+    std::vector<int64_t> p0(np + 1, 0), t0(np + 1, 0), g1o(np + 1, 0), g2o(np + 1, 0), bso(np + 1, 0);
+    int64_t hsz = 0;
+    std::vector<StageJob> stj(np);
+    std::vector<int> stb(np);
+    std::vector<SortChunk> chunks;
+    std::vector<TransformInfo> tinfo;
+    int max_nbins = 1;
+    int sblocks = 0;
+    for (int q = 0; q < np; ++q) {
+        const Engine<R> &e = *parts[q].e;
+        const Core<R> &c = e.core;
+        const int ntr = parts[q].b * e.nact;
+        p0[q + 1] = p0[q] + (int64_t)parts[q].b * e.Mact;
+        t0[q + 1] = t0[q] + ntr;
+        g1o[q + 1] = g1o[q] + (int64_t)ntr * c.n[2] * c.n[1] * c.N[0];
+        g2o[q + 1] = g2o[q] + (c.dim == 3 ? (int64_t)ntr * c.n[2] * c.N[1] * c.N[0] : 0);
+        bso[q + 1] = bso[q] + (int64_t)ntr * (c.g.nbins + 1);
+        max_nbins = std::max(max_nbins, c.g.nbins);
+        StageJob &J = stj[q];
+        std::memset(&J, 0, sizeof(J));
+        J.g = c.g;
+        J.sp = e.sp;
+        J.pos = reinterpret_cast<const float4 *>(parts[q].pos);
+        J.at_ind = e.at_ind.p ? e.at_ind.template as<int4>() : nullptr;
+        J.npts = (int64_t)parts[q].b * e.Mact;
+        J.out0 = p0[q];
+        J.nspan = e.n_tot;
+        J.mact = e.Mact;
+        J.blk0 = sblocks;
+        J.nact = e.nact;
+        for (int a = 0; a < e.nact; ++a) {
+            J.act_src[a] = e.sp_off[e.act[a]];
+            J.act_off[a] = e.act_off[a];
+        }
+        J.act_off[e.nact] = e.act_off[e.nact];
+        stb[q] = sblocks;
+        sblocks += (int)((J.npts + 255) / 256);
+        // sort chunks of every transform (frame fb, active species a)
+        const int64_t CH = c.sort_chunk;
+        for (int fb = 0; fb < parts[q].b; ++fb)
+            for (int a = 0; a < e.nact; ++a) {
+                const int64_t pa = p0[q] + (int64_t)fb * e.Mact + e.act_off[a];
+                const int64_t pb = p0[q] + (int64_t)fb * e.Mact + e.act_off[a + 1];
+                const int nch = (int)((pb - pa + CH - 1) / CH);
+                for (int k = 0; k < nch; ++k) {
+                    SortChunk ch;
+                    ch.p0 = pa + (int64_t)k * CH;
+                    ch.p1 = std::min(pb, pa + (int64_t)(k + 1) * CH);
+                    ch.pbase = pa;
+                    ch.hbase = hsz;
+                    ch.nch = nch;
+                    ch.lc = k;
+                    ch.nbins = c.g.nbins;
+                    ch.keybits = c.keybits();
+                    chunks.push_back(ch);
+                }
+                TransformInfo ti;
+                ti.p0 = pa;
+                ti.p1 = pb;
+                ti.hbase = hsz;
+                ti.bs0 = bso[q] + (int64_t)(fb * e.nact + a) * (c.g.nbins + 1);
+                ti.nch = nch;
+                ti.nbins = c.g.nbins;
+                tinfo.push_back(ti);
+                hsz += (int64_t)c.g.nbins * nch;
+            }
+    }
+    const int64_t npts = p0[np], ntr_all = t0[np];
+    // ---- buffers
+    const size_t nb = (size_t)std::max<int64_t>(npts, 1);
+    ws.rec.ensure(sizeof(PtRec<R>) * nb);
+    ws.srec.ensure(sizeof(PtRec<R>) * nb);
+    ws.key.ensure(sizeof(int) * nb);
+    ws.cstr.ensure(sizeof(Cx<R>) * nb);
+    ws.cs.ensure(sizeof(Cx<R>) * nb);
+    ws.H.ensure(sizeof(int) * (size_t)std::max<int64_t>(hsz, 1));
+    ws.Hs.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(hsz, 1));
+    ws.bsum.ensure(sizeof(int64_t) * (size_t)scan_bsum_size(hsz));
+    ws.binstart.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(bso[np], 1));
+    ws.G1.ensure(sizeof(Cx<R>) * (size_t)std::max<int64_t>(g1o[np], 1));
+    ws.G2.ensure(sizeof(Cx<R>) * (size_t)std::max<int64_t>(g2o[np], 1));
+    ws.bad.ensure(sizeof(int));
+    ws.wc.ensure(sizeof(int) * 8);
+    // ---- spread and column job tables (parts grouped by kernel width)
+    std::vector<int> widths;
+    for (int q = 0; q < np; ++q)
+        if (std::find(widths.begin(), widths.end(), parts[q].e->core.g.w) == widths.end())
+            widths.push_back(parts[q].e->core.g.w);
+    struct SpreadLaunch {
+        int w;
+        size_t o_jobs, o_t0, o_xr, lds;
+        int njobs;
+    };
+    std::vector<SpreadLaunch> sl;
+    std::vector<std::vector<SpreadJob<R>>> sjobs(widths.size());
+    std::vector<std::vector<int>> st0(widths.size()), sxr(widths.size());
+    for (size_t wi = 0; wi < widths.size(); ++wi) {
+        std::vector<double> cost;
+        std::vector<int> ntl;
+        int tile = 0;
+        size_t lds = 0;
+        for (int q = 0; q < np; ++q) {
+            const Engine<R> &e = *parts[q].e;
+            const Core<R> &c = e.core;
+            if (c.g.w != widths[wi]) continue;
+            const int ntr = parts[q].b * e.nact;
+            SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), ws.cs.template as<Cx<R>>(), 0,
+                                         ws.binstart.template as<int64_t>() + bso[q], 1,
+                                         ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
+            J.tile0 = tile;
+            tile += J.ntiles * ntr;
+            sjobs[wi].push_back(J);
+            st0[wi].push_back(J.tile0);
+            cost.push_back(c.tile_cost(e.Mact / std::max(1, e.nact)));
+            ntl.push_back(J.ntiles * ntr);
+            lds = std::max(lds, c.spread_lds_size());
+        }
+        sxr[wi] = xcd_ranges(cost, ntl);
+        sl.push_back(SpreadLaunch{widths[wi], 0, 0, 0, lds, (int)sjobs[wi].size()});
+    }
+    std::vector<ColJob<R>> cyj, czj;
+    std::vector<int> cyb, czb;
+    int cyn = 0, czn = 0;
+    size_t cyl = 0, czl = 0;
+    for (int q = 0; q < np; ++q) {
+        const Engine<R> &e = *parts[q].e;
+        const Core<R> &c = e.core;
+        const int ntr = parts[q].b * e.nact;
+        const int ngrp = e.combine ? parts[q].b : ntr;
+        const int nspg = e.combine ? e.nact : 1;
+        Cx<R> *g1 = ws.G1.template as<Cx<R>>() + g1o[q];
+        Cx<R> *g2 = ws.G2.template as<Cx<R>>() + g2o[q];
+        Cx<R> *out = static_cast<Cx<R> *>(parts[q].amp);
+        const R *wsc = e.combine ? e.wsc.template as<R>() : nullptr;
+        const float *wtab = (e.combine && e.wtab.p) ? e.wtab.template as<float>() : nullptr;
+        if (c.dim == 3) {
+            ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
+            y.blk0 = cyn;
+            cyn += (int)Core<R>::col_blocks(y);
+            cyl = std::max(cyl, colfft_lds<R>(y.cp, false));
+            cyj.push_back(y);
+            cyb.push_back(y.blk0);
+            ColJob<R> z = c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
+            z.blk0 = czn;
+            czn += (int)Core<R>::col_blocks(z);
+            czl = std::max(czl, colfft_lds<R>(z.cp, true));
+            czj.push_back(z);
+            czb.push_back(z.blk0);
+        } else {
+            ColJob<R> z = c.coljob(1, true, nspg, ngrp, g1, out, wsc, wtab);
+            z.blk0 = czn;
+            czn += (int)Core<R>::col_blocks(z);
+            czl = std::max(czl, colfft_lds<R>(z.cp, true));
+            czj.push_back(z);
+            czb.push_back(z.blk0);
+        }
+    }
+    // ---- tables
+    Arena &A = ws.tab;
+    A.begin();
+    const size_t o_stj = A.put(stj), o_stb = A.put(stb), o_ch = A.put(chunks), o_ti = A.put(tinfo);
+    for (size_t wi = 0; wi < sl.size(); ++wi) {
+        sl[wi].o_jobs = A.put(sjobs[wi]);
+        sl[wi].o_t0 = A.put(st0[wi]);
+        sl[wi].o_xr = A.put(sxr[wi]);
+    }
+    const size_t o_cy = A.put(cyj), o_cyb = A.put(cyb), o_cz = A.put(czj), o_czb = A.put(czb);
+    A.upload(st);
+    // ---- launches
+    int ev = tm->begin(ST_STAGE, st);
+    launch_stage_jobs<R>(A.dev<StageJob>(o_stj), A.dev<int>(o_stb), np, sblocks, ws.rec.template as<PtRec<R>>(),
+                         ws.key.template as<int>(), ws.cstr.template as<Cx<R>>(), ws.bad.template as<int>(), st);
+    tm->end(ev, st);
+    ev = tm->begin(ST_SORT, st);
+    const int nch = (int)chunks.size();
+    launch_sort_count(ws.key.template as<int>(), A.dev<SortChunk>(o_ch), nch, max_nbins, ws.H.template as<int>(), st);
+    launch_scan(ws.H.template as<int>(), hsz, ws.bsum.template as<int64_t>(), ws.Hs.template as<int64_t>(), st);
+    launch_sort_scatter<R>(ws.key.template as<int>(), A.dev<SortChunk>(o_ch), nch, max_nbins,
+                           ws.Hs.template as<int64_t>(), ws.rec.template as<PtRec<R>>(),
+                           ws.cstr.template as<Cx<R>>(), ws.srec.template as<PtRec<R>>(),
+                           ws.cs.template as<Cx<R>>(), nullptr, st);
+    launch_binstart(ws.Hs.template as<int64_t>(), A.dev<TransformInfo>(o_ti), (int)ntr_all, max_nbins,
+                    ws.binstart.template as<int64_t>(), st);
+    tm->end(ev, st);
+    ev = tm->begin(ST_SPREAD, st);
+    for (const SpreadLaunch &L : sl)
+        launch_spread_jobs<R>(A.dev<SpreadJob<R>>(L.o_jobs), A.dev<int>(L.o_t0), L.njobs, L.w, A.dev<int>(L.o_xr),
+                              ws.wc.template as<int>(), L.lds, st);
+    tm->end(ev, st);
+    if (cyn > 0) {
+        ev = tm->begin(ST_COLPASS, st);
+        launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cy), A.dev<int>(o_cyb), (int)cyj.size(), cyn, false, cyl, st);
+        tm->end(ev, st);
+    }
+    ev = tm->begin(ST_FINAL, st);
+    launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cz), A.dev<int>(o_czb), (int)czj.size(), czn, true, czl, st);
+    tm->end(ev, st);
+}
 
 }  // namespace mpsq
 

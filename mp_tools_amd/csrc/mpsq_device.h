@@ -238,7 +238,7 @@ __device__ __forceinline__ void stockham_stage(C *buf, int n, int cnt, FDiv dcnt
 // Full forward FFT of the tile (natural order in, natural order out).
 // dcnt = make_fdiv(cnt) (used in col_major mode).
 template <typename C, int E, bool PAD = false>
-__device__ void lds_fft(C *buf, const RadixPlan &rp, int cnt, FDiv dcnt, int es, int ts, const C *__restrict__ tw,
+__device__ __forceinline__ void lds_fft(C *buf, const RadixPlan &rp, int cnt, FDiv dcnt, int es, int ts, const C *__restrict__ tw,
                         bool col_major)
 {
     int Ns = 1;

@@ -54,11 +54,13 @@ struct SortChunk {
     int64_t pbase;    // first point of the chunk's transform
     int64_t hbase;    // start of the transform's count matrix
     int nch, lc;      // chunks of the transform, index of this chunk
+    int nbins, keybits;  // sort bins of the chunk's transform, bits of its keys
 };
 
 struct TransformInfo {
     int64_t p0, p1, hbase;
-    int nch, pad;
+    int64_t bs0;      // first entry of the transform's binstart block (nbins + 1 entries)
+    int nch, nbins;
 };
 
 struct ColPass {
@@ -68,6 +70,61 @@ struct ColPass {
     int nsp;
     int64_t NQ;
     int conj;
+};
+
+// ---------------------------------------------------------------------------
+// Job tables.  One launch of a phase serves any number of jobs (different
+// geometries, frame counts, species); a single-job run is a one-entry table.
+// ---------------------------------------------------------------------------
+constexpr int MAX_JOBS = 256;   // jobs of one launch (tile-id prefix cached in LDS)
+
+// trajectory staging of one job's batch of frames: points [out0, out0+npts)
+// of the shared point buffers, blocks [blk0, blk0 + ceil(npts/256))
+struct StageJob {
+    Geo g;
+    StageParams sp;
+    const float4 *pos;   // first frame of the batch
+    const int4 *at_ind;
+    int64_t npts;        // frames * atoms staged (active species only)
+    int64_t out0;
+    int64_t nspan;       // atoms per frame in pos (n_tot) and active atoms per frame
+    int64_t mact;
+    int blk0;
+    int nact;
+    int64_t act_src[64]; // first atom of active species k in a frame
+    int64_t act_off[65]; // prefix of active species counts
+};
+
+// spread + x FFT of ntr transforms sharing one geometry; tiles
+// [tile0, tile0 + ntiles * ntr) of the launch's tile-id space
+template <typename R> struct SpreadJob {
+    Geo g;
+    RadixPlan rp1;
+    const PtRec<R> *srec;                // sorted records (absolute positions)
+    const typename cplx<R>::T *cs;       // sorted strengths, transform t at + t*c_tstride
+    int64_t c_tstride;
+    const int64_t *binstart;             // transform t's block at + t*bin_tstride*(nbins+1)
+    int bin_tstride;
+    const typename cplx<R>::T *tw1;
+    const R *phinv1;
+    typename cplx<R>::T *G1;             // [ntr][n3][n2][N1]
+    const float *amax;                   // per transform, or nullptr (strengths bounded by 1)
+    int final_conj;
+    int ntr, ntiles, tile0;
+    FDiv dntr, dnty, dby;
+    int lds_tw;                          // byte offset of the twiddles in dynamic LDS
+};
+
+// one column-FFT pass of one job: blocks [blk0, blk0 + ncb*outer*ngroups)
+template <typename R> struct ColJob {
+    ColPass cp;
+    RadixPlan rp;
+    const typename cplx<R>::T *in;
+    typename cplx<R>::T *out;
+    const typename cplx<R>::T *tw;
+    const R *phinv, *wsc;
+    const float *wtab;
+    int ncb, ngroups, blk0;
 };
 
 // S(Q,w) reduction parameters
@@ -88,16 +145,16 @@ template <typename R>
 void launch_fold(const Geo &g, int64_t M, const void *x, const void *y, const void *z, bool in_f32, PtRec<R> *rec,
                  int *key, int *bad, hipStream_t st);
 template <typename R>
-void launch_stage(const Geo &g, const StageParams &sp, int64_t npts, const float4 *pos, const int4 *at_ind,
-                  PtRec<R> *rec, int *key, typename cplx<R>::T *cstr, int *bad, hipStream_t st);
-void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int nbins, int *H, hipStream_t st);
+void launch_stage_jobs(const StageJob *jobs, const int *blk0s, int njobs, int nblocks, PtRec<R> *rec, int *key,
+                       typename cplx<R>::T *cstr, int *bad, hipStream_t st);
+void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int max_nbins, int *H, hipStream_t st);
 void launch_scan(const int *H, int64_t n, int64_t *bsum, int64_t *Hs, hipStream_t st);
 int64_t scan_bsum_size(int64_t n);
 template <typename R>
-void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int nbins, int keybits,
-                         const int64_t *Hs, const PtRec<R> *rec, const typename cplx<R>::T *cin, PtRec<R> *srec,
+void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int max_nbins, const int64_t *Hs,
+                         const PtRec<R> *rec, const typename cplx<R>::T *cin, PtRec<R> *srec,
                          typename cplx<R>::T *cout, int *perm, hipStream_t st);
-void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int nbins, int64_t *binstart,
+void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int max_nbins, int64_t *binstart,
                      hipStream_t st);
 template <typename R>
 void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::T *c, typename cplx<R>::T *cs,
@@ -107,19 +164,20 @@ int spread_max_runs();
 int spread_max_tile_b();
 template <typename R> size_t spread_lds(int rows, int n1, int w);
 template <typename R> int colfft_tile_elems(bool final);
-// amax[t] bounds |Re c|, |Im c| of transform t's strengths (FP32 form only;
-// nullptr: strengths are known to be bounded by 1)
+// Spread + x FFT of every tile of a job table in one persistent launch.
+// Workgroups run on XCD blockIdx.x % 8 and take tiles of their XCD's range
+// [xr[x], xr[x+1]) of the tile-id space from the counter wc[x] (zeroed by
+// this call).  All jobs share the kernel width w; lds = max over jobs of
+// spread_lds.
 template <typename R>
-void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
-                        const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
-                        const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
-                        const float *amax, hipStream_t st);
+void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
+                        size_t lds, hipStream_t st);
 // amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)
 void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t st);
+template <typename R> size_t colfft_lds(const ColPass &cp, bool final);
 template <typename R>
-void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
-                   typename cplx<R>::T *out, const typename cplx<R>::T *tw, const R *phinv, const R *wsc,
-                   const float *wtab, bool final, hipStream_t st);
+void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int nblocks, bool final, size_t lds,
+                        hipStream_t st);
 
 // --- launch helpers (sqw_kernels.hip) ---
 template <typename R> int sqw_tile_elems();

@@ -98,18 +98,47 @@ __global__ void k_fold(Geo g, int64_t M, const Tin *__restrict__ x, const Tin *_
 // Dropped atoms keep their slot with strength 0, which leaves every
 // amplitude unchanged and keeps the per-transform point counts fixed.
 // ---------------------------------------------------------------------------
+// job j's blocks are [jobs[j].blk0, jobs[j+1].blk0): point p of the job is
+// active atom a = p % mact of frame p / mact; active species k holds atoms
+// [act_off[k], act_off[k+1]) of that numbering, taken from frame atoms
+// act_src[k] + (a - act_off[k]) (species-major storage)
+__device__ __forceinline__ int find_job(const int *__restrict__ blk0s, int njobs, int b)
+{
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (blk0s[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    // uniform by construction (b is the block index); saying so keeps the
+    // job's fields in scalar registers
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+
 template <typename R>
-__global__ void k_stage_traj(Geo g, StageParams sp, int64_t npts, const float4 *__restrict__ pos,
-                             const int4 *__restrict__ at_ind, PtRec<R> *__restrict__ rec,
-                             int *__restrict__ key, typename cplx<R>::T *__restrict__ cstr,
-                             int *__restrict__ bad_flag)
+__global__ __launch_bounds__(256) void k_stage_traj(const StageJob *__restrict__ jobs, const int *__restrict__ blk0s,
+                                                    int njobs, PtRec<R> *__restrict__ rec_all,
+                                                    int *__restrict__ key_all,
+                                                    typename cplx<R>::T *__restrict__ cstr_all,
+                                                    int *__restrict__ bad_flag)
 {
 #pragma clang fp contract(off)
     using C = typename cplx<R>::T;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= npts) return;
-    const int64_t a = p % sp.n_tot;
-    const float4 r = pos[p];
+    const int jb = find_job(blk0s, njobs, blockIdx.x);
+    const StageJob &J = jobs[jb];
+    const Geo &g = J.g;
+    const StageParams &sp = J.sp;
+    const int64_t p = (int64_t)(blockIdx.x - J.blk0) * blockDim.x + threadIdx.x;
+    if (p >= J.npts) return;
+    const int64_t fb = p / J.mact, aa = p - fb * J.mact;
+    int k = 0;
+    while (k + 1 < J.nact && J.act_off[k + 1] <= aa) ++k;
+    const int64_t a = J.act_src[k] + (aa - J.act_off[k]);
+    const float4 r = J.pos[fb * J.nspan + a];
+    const int4 *at_ind = J.at_ind;
+    PtRec<R> *rec = rec_all + J.out0;
+    int *key = key_all + J.out0;
+    C *cstr = cstr_all + J.out0;
     bool keep = true;
     if (sp.vacancy_check) keep = (r.x != 0.0f) || (r.y != 0.0f) || (r.z != 0.0f);
     if (sp.select_mode == 1 && at_ind) {  // CELL window on cell indices
@@ -182,11 +211,11 @@ __global__ void k_stage_traj(Geo g, StageParams sp, int64_t npts, const float4 *
 // ---------------------------------------------------------------------------
 // stable bin sort
 // ---------------------------------------------------------------------------
-__global__ void k_sort_count(const int *__restrict__ key, const SortChunk *__restrict__ chunks,
-                             int nbins, int *__restrict__ H)
+__global__ void k_sort_count(const int *__restrict__ key, const SortChunk *__restrict__ chunks, int *__restrict__ H)
 {
     extern __shared__ int hist[];
     const SortChunk ch = chunks[blockIdx.x];
+    const int nbins = ch.nbins;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
     __syncthreads();
     for (int64_t p = ch.p0 + threadIdx.x; p < ch.p1; p += blockDim.x)
@@ -266,7 +295,7 @@ __global__ __launch_bounds__(256) void k_scan_final(const int *__restrict__ H, i
 // (destination of its (bin, chunk) cell).
 template <typename R>
 __global__ __launch_bounds__(256) void k_sort_scatter(const int *__restrict__ key, const SortChunk *__restrict__ chunks,
-                                                      int nbins, int keybits, const int64_t *__restrict__ Hs,
+                                                      const int64_t *__restrict__ Hs,
                                                       const PtRec<R> *__restrict__ rec,
                                                       const typename cplx<R>::T *__restrict__ cin,
                                                       PtRec<R> *__restrict__ srec,
@@ -275,6 +304,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *__restrict__ ke
 {
     extern __shared__ int cnt[];
     const SortChunk ch = chunks[blockIdx.x];
+    const int nbins = ch.nbins, keybits = ch.keybits;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) cnt[b] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -309,15 +339,15 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const int *__restrict__ ke
     }
 }
 
-// binstart[t][b] = first sorted slot of bin b of transform t; [t][nbins] = end
-__global__ void k_binstart(const int64_t *__restrict__ Hs, const TransformInfo *__restrict__ tinfo, int ntr,
-                           int nbins, int64_t *__restrict__ binstart)
+// binstart[bs0 + b] = first sorted slot of bin b of transform t;
+// [bs0 + nbins] = end (grid: x over bins, y over transforms)
+__global__ void k_binstart(const int64_t *__restrict__ Hs, const TransformInfo *__restrict__ tinfo,
+                           int64_t *__restrict__ binstart)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)ntr * (nbins + 1)) return;
-    const int t = (int)(i / (nbins + 1)), b = (int)(i % (nbins + 1));
-    const TransformInfo ti = tinfo[t];
-    binstart[i] = (b < nbins && ti.nch > 0) ? Hs[ti.hbase + (int64_t)b * ti.nch] : ti.p1;
+    const TransformInfo ti = tinfo[blockIdx.y];
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > ti.nbins) return;
+    binstart[ti.bs0 + b] = (b < ti.nbins && ti.nch > 0) ? Hs[ti.hbase + (int64_t)b * ti.nch] : ti.p1;
 }
 
 // strengths of the plan API: cs[tr][k] = c[tr][perm[k]] (conjugated for iflag > 0)
@@ -365,31 +395,33 @@ __global__ void k_absmax(int64_t M, const float2 *__restrict__ c, unsigned *__re
 // (conjugated for iflag > 0).
 // ---------------------------------------------------------------------------
 template <typename R, int E, int NT, bool FINAL>
-__global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const typename cplx<R>::T *__restrict__ in,
-                                               typename cplx<R>::T *__restrict__ out,
-                                               const typename cplx<R>::T *__restrict__ tw,
-                                               const R *__restrict__ phinv, const R *__restrict__ wsc,
-                                               const float *__restrict__ wtab)
+__global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ jobs, const int *__restrict__ blk0s,
+                                               int njobs)
 {
     using C = typename cplx<R>::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    const ColJob<R> &J = jobs[find_job(blk0s, njobs, blockIdx.x)];
+    const ColPass &cp = J.cp;
     C *tile = reinterpret_cast<C *>(smem_raw);
     const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
     C *tw_l = tile + n * cols;  // twiddles in LDS
     C *acc = tw_l + n;          // FINAL: species sum of the kept modes, [N][cols]
-    for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = tw[i];
-    const int ncb = (inner + cols - 1) / cols;
-    const int o = blockIdx.x / ncb, cb = blockIdx.x - o * ncb;
+    for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = J.tw[i];
+    const int ncb = J.ncb;
+    const int local = blockIdx.x - J.blk0;
+    const int grp = local / (ncb * cp.outer);
+    const int rem = local - grp * ncb * cp.outer;
+    const int o = rem / ncb, cb = rem - o * ncb;
     const int col0 = cb * cols;
     const int ncol = min(cols, inner - col0);
     const int h = N / 2;
     const int nsp = FINAL ? cp.nsp : 1;
     for (int s = 0; s < nsp; ++s) {
-        const int t = FINAL ? blockIdx.y * cp.nsp + s : blockIdx.y;
+        const int t = FINAL ? grp * cp.nsp + s : grp;
         {
             // all E loads of a thread are issued before the first LDS store;
             // tile element i = threadIdx.x + k * NT is row i / cols, column i % cols
-            const C *src = in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
+            const C *src = J.in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
             C v[E];
 #pragma unroll
             for (int k = 0; k < E; ++k) {
@@ -404,24 +436,24 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
             }
         }
         __syncthreads();
-        lds_fft<C, E>(tile, rp, cols, cp.dcols, cols, 1, tw_l, true);
+        lds_fft<C, E>(tile, J.rp, cols, cp.dcols, cols, 1, tw_l, true);
         if (!FINAL) {
-            C *dst = out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
+            C *dst = J.out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
             for (int i = threadIdx.x; i < N * cols; i += NT) {
                 const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 if (c >= ncol) continue;
                 int f = kk - h;
                 if (f < 0) f += n;
-                dst[(int64_t)kk * inner + c] = cscale(tile[f * cols + c], phinv[kk]);
+                dst[(int64_t)kk * inner + c] = cscale(tile[f * cols + c], J.phinv[kk]);
             }
         } else {
             for (int i = threadIdx.x; i < N * cols; i += NT) {
                 const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 int f = kk - h;
                 if (f < 0) f += n;
-                R wv = phinv[kk];
-                if (wtab) wv *= (c < ncol) ? (R)wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c] : (R)0;
-                else if (wsc) wv *= wsc[s];
+                R wv = J.phinv[kk];
+                if (J.wtab) wv *= (c < ncol) ? (R)J.wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c] : (R)0;
+                else if (J.wsc) wv *= J.wsc[s];
                 const C x = cscale(tile[f * cols + c], wv);
                 acc[i] = (s == 0) ? x : cadd(acc[i], x);
             }
@@ -429,7 +461,7 @@ __global__ __launch_bounds__(NT) void k_colfft(ColPass cp, RadixPlan rp, const t
         __syncthreads();
     }
     if (FINAL) {
-        C *dst = out + (int64_t)blockIdx.y * cp.out_tstride + (int64_t)o * N * inner + col0;
+        C *dst = J.out + (int64_t)grp * cp.out_tstride + (int64_t)o * N * inner + col0;
         for (int i = threadIdx.x; i < N * cols; i += NT) {
             const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
             if (c >= ncol) continue;
@@ -468,19 +500,18 @@ void launch_fold(const Geo &g, int64_t M, const void *x, const void *y, const vo
 }
 
 template <typename R>
-void launch_stage(const Geo &g, const StageParams &sp, int64_t npts, const float4 *pos, const int4 *at_ind,
-                  PtRec<R> *rec, int *key, typename cplx<R>::T *cstr, int *bad, hipStream_t st)
+void launch_stage_jobs(const StageJob *jobs, const int *blk0s, int njobs, int nblocks, PtRec<R> *rec, int *key,
+                       typename cplx<R>::T *cstr, int *bad, hipStream_t st)
 {
-    const int NT = 256;
-    if (npts == 0) return;
-    hipLaunchKernelGGL((k_stage_traj<R>), dim3((unsigned)((npts + NT - 1) / NT)), dim3(NT), 0, st, g, sp, npts, pos,
-                       at_ind, rec, key, cstr, bad);
+    if (nblocks <= 0) return;
+    hipLaunchKernelGGL((k_stage_traj<R>), dim3((unsigned)nblocks), dim3(256), 0, st, jobs, blk0s, njobs, rec, key,
+                       cstr, bad);
 }
 
-void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int nbins, int *H, hipStream_t st)
+void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int max_nbins, int *H, hipStream_t st)
 {
     if (nchunks == 0) return;
-    hipLaunchKernelGGL(k_sort_count, dim3(nchunks), dim3(256), nbins * sizeof(int), st, key, chunks, nbins, H);
+    hipLaunchKernelGGL(k_sort_count, dim3(nchunks), dim3(256), max_nbins * sizeof(int), st, key, chunks, H);
 }
 
 void launch_scan(const int *H, int64_t n, int64_t *bsum, int64_t *Hs, hipStream_t st)
@@ -497,21 +528,21 @@ void launch_scan(const int *H, int64_t n, int64_t *bsum, int64_t *Hs, hipStream_
 int64_t scan_bsum_size(int64_t n) { return (n + 256LL * 16 - 1) / (256LL * 16) + 1; }
 
 template <typename R>
-void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int nbins, int keybits,
-                         const int64_t *Hs, const PtRec<R> *rec, const typename cplx<R>::T *cin, PtRec<R> *srec,
+void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int max_nbins, const int64_t *Hs,
+                         const PtRec<R> *rec, const typename cplx<R>::T *cin, PtRec<R> *srec,
                          typename cplx<R>::T *cout, int *perm, hipStream_t st)
 {
     if (nchunks == 0) return;
-    hipLaunchKernelGGL((k_sort_scatter<R>), dim3(nchunks), dim3(256), nbins * sizeof(int), st, key, chunks, nbins,
-                       keybits, Hs, rec, cin, srec, cout, perm);
+    hipLaunchKernelGGL((k_sort_scatter<R>), dim3(nchunks), dim3(256), max_nbins * sizeof(int), st, key, chunks, Hs,
+                       rec, cin, srec, cout, perm);
 }
 
-void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int nbins, int64_t *binstart,
+void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int max_nbins, int64_t *binstart,
                      hipStream_t st)
 {
-    const int64_t n = (int64_t)ntr * (nbins + 1);
-    hipLaunchKernelGGL(k_binstart, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Hs, tinfo, ntr, nbins,
-                       binstart);
+    if (ntr <= 0) return;
+    hipLaunchKernelGGL(k_binstart, dim3((unsigned)((max_nbins + 1 + 255) / 256), (unsigned)ntr), dim3(256), 0, st, Hs,
+                       tinfo, binstart);
 }
 
 template <typename R>
@@ -537,44 +568,46 @@ void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t
 
 template <typename R> int colfft_tile_elems(bool final) { return Tiles<R>::E * colfft_nt(final); }
 
-template <typename R>
-void launch_colfft(const ColPass &cp, const RadixPlan &rp, int ngroups, const typename cplx<R>::T *in,
-                   typename cplx<R>::T *out, const typename cplx<R>::T *tw, const R *phinv, const R *wsc,
-                   const float *wtab, bool final, hipStream_t st)
+// LDS of one column pass of a job: tile + twiddles (+ species sum for the final pass)
+template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
 {
+    return ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) * sizeof(typename cplx<R>::T);
+}
+
+template <typename R>
+void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int nblocks, bool final, size_t lds,
+                        hipStream_t st)
+{
+    if (nblocks <= 0) return;
     constexpr int E = Tiles<R>::E, NTF = colfft_nt(true), NTY = colfft_nt(false);
-    const int ncb = (cp.inner + cp.cols - 1) / cp.cols;
-    const size_t lds = ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) *
-                       sizeof(typename cplx<R>::T);
-    dim3 grid(ncb * cp.outer, ngroups);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTF, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTY, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         attr = true;
     }
     if (final)
-        hipLaunchKernelGGL((k_colfft<R, E, NTF, true>), grid, dim3(NTF), lds, st, cp, rp, in, out, tw, phinv, wsc,
-                           wtab);
+        hipLaunchKernelGGL((k_colfft<R, E, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
+                           njobs);
     else
-        hipLaunchKernelGGL((k_colfft<R, E, NTY, false>), grid, dim3(NTY), lds, st, cp, rp, in, out, tw, phinv, wsc,
-                           wtab);
+        hipLaunchKernelGGL((k_colfft<R, E, NTY, false>), dim3((unsigned)nblocks), dim3(NTY), lds, st, jobs, blk0s,
+                           njobs);
 }
 
 #define MPSQ_INST(R)                                                                                                  \
     template void launch_fold<R>(const Geo &, int64_t, const void *, const void *, const void *, bool, PtRec<R> *,    \
                                  int *, int *, hipStream_t);                                                          \
-    template void launch_stage<R>(const Geo &, const StageParams &, int64_t, const float4 *, const int4 *,           \
-                                  PtRec<R> *, int *, cplx<R>::T *, int *, hipStream_t);                               \
-    template void launch_sort_scatter<R>(const int *, const SortChunk *, int, int, int, const int64_t *,             \
+    template void launch_stage_jobs<R>(const StageJob *, const int *, int, int, PtRec<R> *, int *, cplx<R>::T *,     \
+                                       int *, hipStream_t);                                                           \
+    template void launch_sort_scatter<R>(const int *, const SortChunk *, int, int, const int64_t *,                  \
                                          const PtRec<R> *, const cplx<R>::T *, PtRec<R> *, cplx<R>::T *, int *,       \
                                          hipStream_t);                                                                \
     template void launch_gather<R>(int64_t, int, const int *, const cplx<R>::T *, cplx<R>::T *, int, hipStream_t);    \
     template int colfft_tile_elems<R>(bool);                                                                          \
-    template void launch_colfft<R>(const ColPass &, const RadixPlan &, int, const cplx<R>::T *, cplx<R>::T *,        \
-                                   const cplx<R>::T *, const R *, const R *, const float *, bool, hipStream_t);
+    template size_t colfft_lds<R>(const ColPass &, bool);                                                             \
+    template void launch_colfft_jobs<R>(const ColJob<R> *, const int *, int, int, bool, size_t, hipStream_t);
 
 MPSQ_INST(float)
 MPSQ_INST(double)

@@ -161,71 +161,75 @@ __device__ __forceinline__ void stage_any(int R, int n, int cnt, FDiv dnb, FDiv 
     }
 }
 
-// Tile id sequence of a persistent workgroup: workgroup b runs on XCD b % 8
-// (round-robin dispatch of the resident grid) and takes the tiles
-// x*per + b/8 + k*(grid/8) of its XCD's contiguous range [x*per, (x+1)*per),
-// so the 2-per-CU workgroups of one XCD always work on neighbouring tiles.
-// Tile id = (tile position) * ntr + transform: every XCD gets the same mix of
-// transforms (species differ 3x in point count; a transform-major order left
-// XCDs unevenly loaded), and the tiles in flight on one XCD still cover a few
-// neighbouring positions, whose candidates they share through its L2.
-struct TileSeq {
-    int cur, step, end;
-    __device__ TileSeq(int ntot)
-    {
-        const int x = blockIdx.x & 7, li = blockIdx.x >> 3, nl = gridDim.x >> 3;
-        const int per = (ntot + 7) >> 3;
-        cur = x * per + li;
-        step = nl;
-        end = min(ntot, (x + 1) * per);
+// binary search of the job owning tile id (tile0s ascending, tile0s[0] = 0)
+__device__ __forceinline__ int job_of(const int *__restrict__ tile0s, int njobs, int id)
+{
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile0s[mid] <= id) lo = mid;
+        else hi = mid;
     }
-};
+    return __builtin_amdgcn_readfirstlane(lo);  // uniform: id is the workgroup's tile
+}
 
+// Persistent spread + x FFT over the tiles of a job table.
+//
+// Tile ids: job-major; inside a job, id - tile0 = (tile position) * ntr +
+// transform, so the tiles taken one after another cover every transform of
+// a few neighbouring positions (their candidates overlap and come from the
+// XCD's L2), and the species (3x different point counts) are spread evenly.
+// Workgroup b runs on XCD x = b % 8 (round-robin dispatch of the resident
+// grid) and takes the next tile of its XCD's range [xr[x], xr[x+1]) from the
+// counter wc[x]; the host cuts the ranges at equal estimated cost.  While a
+// tile is spread, wave 0 already holds the next tile's id and loads its
+// candidate runs; while it is transformed, every lane already loads its first
+// candidate of the next tile.
 template <typename R, int E, int NT, int W>
-__global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, const PtRec<R> *__restrict__ srec,
-                                                    const typename cplx<R>::T *__restrict__ cs, int64_t c_tstride,
-                                                    const int64_t *__restrict__ binstart, int bin_tstride,
-                                                    const typename cplx<R>::T *__restrict__ tw1,
-                                                    const R *__restrict__ phinv1, typename cplx<R>::T *__restrict__ G1,
-                                                    int final_conj, int ntr, int ntiles_all, FDiv dntr,
-                                                    FDiv dnty, FDiv dby, const float *__restrict__ amax)
+__global__ __launch_bounds__(NT, 4) void k_spread_xfft(const SpreadJob<R> *__restrict__ jobs,
+                                                       const int *__restrict__ tile0s, int njobs,
+                                                       const int *__restrict__ xr, int *__restrict__ wc)
 {
     using C = typename cplx<R>::T;
     constexpr bool FIX = (sizeof(R) == 4);
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int64_t rbeg[MAX_RUNS];
-    __shared__ int rpre[MAX_RUNS + 1];
-    __shared__ float s_amax;
+    __shared__ int64_t rbeg[2][MAX_RUNS];  // candidate runs, double-buffered by tile parity
+    __shared__ int rpre[2][MAX_RUNS + 1];
+    __shared__ float s_amax[2];
+    __shared__ int s_next;
 
-    const int n1 = g.n1, w = g.w;
-    const int RS = spread_row_stride(n1, W);
-    const int RSF = padded_len(n1);
-    const int by = (g.dim >= 2) ? g.by : 1;
-    const int bz = (g.dim >= 3) ? g.bz : 1;
-    const int rows = by * bz;
-    const R beta = (R)g.beta, ic = (R)g.ic;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int N1 = g.N1, h1 = N1 / 2;
-
+    const int xcd = blockIdx.x & 7;
+    const int xbeg = xr[xcd], xend = xr[xcd + 1];
     unsigned long long *accu = reinterpret_cast<unsigned long long *>(smem_raw);
     double *accd = reinterpret_cast<double *>(smem_raw);
     C *tile = reinterpret_cast<C *>(smem_raw);
-    const int plane = rows * RS;
-    const size_t accb = (size_t)2 * plane * 8, fftb = (size_t)rows * RSF * sizeof(C);
-    C *tw = reinterpret_cast<C *>(smem_raw + ((accb > fftb ? accb : fftb) + 15) / 16 * 16);
-    for (int i = threadIdx.x; i < n1; i += NT) tw[i] = tw1[i];
 
-    // decode a tile id: transform, first row y0, first plane z0
-    auto decode = [&](int id, int &t, int &y0, int &z0) {
-        const int ti = (int)fdiv((uint32_t)id, dntr);
-        t = id - ti * ntr;
-        const int tz = (int)fdiv((uint32_t)ti, dnty);
-        y0 = (ti - tz * (g.dim >= 2 ? g.n2 / by : 1)) * by;
+    // geometry of a job's tile: transform t, first row y0, first plane z0
+    auto decode = [&](const SpreadJob<R> &J, int id, int &t, int &y0, int &z0) {
+        const int loc = id - J.tile0;
+        const int ti = (int)fdiv((uint32_t)loc, J.dntr);
+        t = loc - ti * J.ntr;
+        const int by = (J.g.dim >= 2) ? J.g.by : 1, bz = (J.g.dim >= 3) ? J.g.bz : 1;
+        const int tz = (int)fdiv((uint32_t)ti, J.dnty);
+        y0 = (ti - tz * ((J.g.dim >= 2) ? J.g.n2 / by : 1)) * by;
         z0 = tz * bz;
     };
-    // wave 0: the candidate runs of a tile (lane e = 2 * z-bin row + part):
-    // start and size, loaded into registers
-    auto runs_load = [&](int t, int y0, int z0, int &nruns, int64_t &b0, int &cnt) {
+    // number of candidate runs of a tile: two per visited z-bin row
+    auto nruns_of = [&](const SpreadJob<R> &J, int z0) {
+        const Geo &g = J.g;
+        int nzb = 1;
+        if (g.dim >= 3) {
+            const int zlo = floordiv(z0 - (g.w - 1), g.BZ);
+            nzb = floordiv(z0 + g.bz - 1, g.BZ) - zlo + 1;
+            if (nzb >= g.nbZ) nzb = g.nbZ;
+        }
+        return 2 * nzb;
+    };
+    // wave 0: start and size of run `lane` of a tile (registers)
+    auto runs_load = [&](const SpreadJob<R> &J, int t, int y0, int z0, int &nruns, int64_t &b0, int &cnt) {
+        const Geo &g = J.g;
+        const int w = g.w, by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
         int ylo = 0, nyb = 1, zlo = 0, nzb = 1;
         if (g.dim >= 2) {
             ylo = floordiv(y0 - (w - 1), g.BY);
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
         b0 = 0;
         cnt = 0;
         if (lane < nruns) {
-            const int64_t *bs = binstart + (int64_t)t * bin_tstride * (g.nbins + 1);
+            const int64_t *bs = J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1);
             const int k = lane >> 1, part = lane & 1;
             const int zb = posmod(zlo + k, g.nbZ);
             int ya = 0, yb = 0;  // bin range [ya, yb) of this part
@@ -278,8 +282,8 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
             }
         }
     };
-    // wave 0: publish the runs (inclusive scan of the sizes)
-    auto runs_store = [&](int nruns, int64_t b0, int cnt, float am) {
+    // wave 0: publish the runs (inclusive scan of the sizes) in buffer pb
+    auto runs_store = [&](int pb, int nruns, int64_t b0, int cnt, float am) {
         int x = cnt;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
@@ -287,77 +291,88 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
             if (lane >= o) x += y;
         }
         if (lane < nruns) {
-            rbeg[lane] = b0;
-            rpre[lane + 1] = x;
+            rbeg[pb][lane] = b0;
+            rpre[pb][lane + 1] = x;
         }
         if (lane == 0) {
-            rpre[0] = 0;
-            s_amax = am;
+            rpre[pb][0] = 0;
+            s_amax[pb] = am;
         }
     };
+    auto load_tw = [&](const SpreadJob<R> &J) {
+        C *tw = reinterpret_cast<C *>(smem_raw + J.lds_tw);
+        for (int i = threadIdx.x; i < J.g.n1; i += NT) tw[i] = J.tw1[i];
+    };
 
-    TileSeq seq(ntiles_all);
-    if (seq.cur >= seq.end) return;
+    // ---- first tile
+    if (threadIdx.x == 0) s_next = xbeg + atomicAdd(wc + xcd, 1);
+    __syncthreads();
+    int cur = s_next;
+    if (cur >= xend) return;
+    int j = job_of(tile0s, njobs, cur);
     int t, y0, z0;
-    decode(seq.cur, t, y0, z0);
-    int nruns = 0;
+    decode(jobs[j], cur, t, y0, z0);
+    load_tw(jobs[j]);
+    int pb = 0;  // run buffer of the current tile
     if (wv == 0) {
+        int nr, cnt;
         int64_t b0;
-        int cnt;
-        runs_load(t, y0, z0, nruns, b0, cnt);
-        runs_store(nruns, b0, cnt, amax ? amax[t] : 1.0f);
+        runs_load(jobs[j], t, y0, z0, nr, b0, cnt);
+        runs_store(pb, nr, b0, cnt, jobs[j].amax ? jobs[j].amax[t] : 1.0f);
     }
     __syncthreads();
 
-    // candidate fetch (runs of the tile currently published)
+    // candidate fetch from the runs currently published
     PtRec<R> pr;
     C cv;
     bool have = false;
     int e = 0, total = 0;
-    auto fetch = [&](int v, int tt) {
+    auto fetch = [&](const SpreadJob<R> &J, int b, int v, int tt) {
         have = v < total;
         if (have) {
-            while (rpre[e + 1] <= v) ++e;
-            const int64_t p = rbeg[e] + (v - rpre[e]);
-            pr = srec[p];
-            cv = cs[(int64_t)tt * c_tstride + p];
+            while (rpre[b][e + 1] <= v) ++e;
+            const int64_t p = rbeg[b][e] + (v - rpre[b][e]);
+            pr = J.srec[p];
+            cv = J.cs[(int64_t)tt * J.c_tstride + p];
         }
     };
-    // number of runs of a tile (wave 0 knows it from runs_load; every
-    // thread needs it to read the tile's candidate total rpre[nruns])
-    auto nruns_of = [&](int yy0, int zz0) {
-        int nzb = 1;
-        if (g.dim >= 3) {
-            const int zlo = floordiv(zz0 - (w - 1), g.BZ);
-            nzb = floordiv(zz0 + bz - 1, g.BZ) - zlo + 1;
-            if (nzb >= g.nbZ) nzb = g.nbZ;
-        }
-        (void)yy0;
-        return 2 * nzb;
-    };
-    total = rpre[nruns_of(y0, z0)];
-    e = 0;
-    fetch(wv * 64 + lane, t);
+    total = rpre[pb][nruns_of(jobs[j], z0)];
+    fetch(jobs[j], pb, wv * 64 + lane, t);
 
     for (;;) {
-        // ---- tile (t, y0, z0): zero the accumulator planes
+        const SpreadJob<R> &J = jobs[j];
+        const Geo &g = J.g;
+        const int n1 = g.n1, w = g.w;
+        const int RS = spread_row_stride(n1, W);
+        const int RSF = padded_len(n1);
+        const int by = (g.dim >= 2) ? g.by : 1;
+        const int bz = (g.dim >= 3) ? g.bz : 1;
+        const int rows = by * bz;
+        const int plane = rows * RS;
+        const R beta = (R)g.beta, ic = (R)g.ic;
+        const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
+
+        // ---- zero the accumulator planes; take the next tile's id
         {
             uint4 *a4 = reinterpret_cast<uint4 *>(smem_raw);
             for (int i = threadIdx.x; i < plane; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
         }
-        const float am = s_amax;
+        const float am = s_amax[pb];
         __syncthreads();
-        const int next = seq.cur + seq.step;
-        const bool has_next = next < seq.end;
-        int tn = 0, y0n = 0, z0n = 0;
-        if (has_next) decode(next, tn, y0n, z0n);
-        // wave 0: the next tile's runs are in flight during this tile's spread
-        int nruns_n = 0, cnt_n = 0;
-        int64_t b0_n = 0;
-        float am_n = 1.0f;
-        if (wv == 0 && has_next) {
-            runs_load(tn, y0n, z0n, nruns_n, b0_n, cnt_n);
-            if (amax) am_n = amax[tn];
+        if (wv == 0) {
+            // the next tile's id and runs (other buffer) while this one is spread
+            int nxt = 0;
+            if (lane == 0) nxt = xbeg + atomicAdd(wc + xcd, 1);
+            nxt = __shfl(nxt, 0, 64);
+            if (lane == 0) s_next = nxt;
+            if (nxt < xend) {
+                int jn, tn, y0n, z0n, nr, cnt;
+                int64_t b0;
+                jn = job_of(tile0s, njobs, nxt);
+                decode(jobs[jn], nxt, tn, y0n, z0n);
+                runs_load(jobs[jn], tn, y0n, z0n, nr, b0, cnt);
+                runs_store(pb ^ 1, nr, b0, cnt, jobs[jn].amax ? jobs[jn].amax[tn] : 1.0f);
+            }
         }
 
         // fixed-point scale 2^q: total * amax * 2^q < 2^45
@@ -373,20 +388,20 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
 
         // ---- spread: every wave walks candidates v = wv*64 + lane + k*NT
         for (int v0 = wv * 64; v0 < total; v0 += NT) {
-            const PtRec<R> cur = pr;
+            const PtRec<R> cur_p = pr;
             const C cc = cv;
             const bool ok = have;
-            fetch(v0 + NT + lane, t);
+            fetch(J, pb, v0 + NT + lane, t);
             if (!ok) continue;
             int fy = 0, ny = 1, my = 0, fz = 0, nz = 1, mz = 0;
-            if (g.dim >= 2) overlap(cur.ly, y0, by, w, g.n2, fy, ny, my);
-            if (g.dim >= 3) overlap(cur.lz, z0, bz, w, g.n3, fz, nz, mz);
+            if (g.dim >= 2) overlap(cur_p.ly, y0, by, w, g.n2, fy, ny, my);
+            if (g.dim >= 3) overlap(cur_p.lz, z0, bz, w, g.n3, fz, nz, mz);
             if (ny <= 0 || nz <= 0) continue;
             double kxd[W];
             R kx[W];
 #pragma unroll
             for (int m = 0; m < W; ++m) {
-                kx[m] = es_kernel(cur.dx + (R)m, beta, ic);
+                kx[m] = es_kernel(cur_p.dx + (R)m, beta, ic);
                 kxd[m] = (double)kx[m] * fscale;
             }
             R ky[MAX_TILE_B];
@@ -394,11 +409,11 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
             for (int iy = 0; iy < MAX_TILE_B; ++iy) {
                 ky[iy] = (R)0;
                 if (iy < by)
-                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(cur.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
+                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(cur_p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
             }
             for (int iz = 0; iz < nz; ++iz) {
-                const R wz = (g.dim >= 3) ? es_kernel(cur.dz + (R)(mz + iz), beta, ic) : (R)1;
-                const int rbase = ((fz + iz) * by + fy) * RS + cur.lx;
+                const R wz = (g.dim >= 3) ? es_kernel(cur_p.dz + (R)(mz + iz), beta, ic) : (R)1;
+                const int rbase = ((fz + iz) * by + fy) * RS + cur_p.lx;
 #pragma unroll
                 for (int iy = 0; iy < MAX_TILE_B; ++iy) {
                     if (iy < by && iy < ny) {
@@ -433,8 +448,7 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
         }
         __syncthreads();
 
-        // ---- publish the next tile's runs (wave 0), fold the ghost cells
-        if (wv == 0 && has_next) runs_store(nruns_n, b0_n, cnt_n, am_n);
+        // ---- fold the ghost cells
         for (int i = threadIdx.x; i < 2 * rows * W; i += NT) {
             const int pl = i / (rows * W), rr = (i / W) % rows, x = i % W;
             const int a = pl * plane + rr * RS;
@@ -443,16 +457,22 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
         }
         __syncthreads();
         // the next tile's first candidates are in flight during this FFT
-        const int total_n = has_next ? rpre[nruns_of(y0n, z0n)] : 0;
-        {
+        const int nxt = s_next;
+        const bool has_next = nxt < xend;
+        int jn = j, tn = 0, y0n = 0, z0n = 0, total_n = 0;
+        if (has_next) {
+            jn = job_of(tile0s, njobs, nxt);
+            decode(jobs[jn], nxt, tn, y0n, z0n);
+            total_n = rpre[pb ^ 1][nruns_of(jobs[jn], z0n)];
             const int tot_cur = total;
             total = total_n;
             e = 0;
-            fetch(wv * 64 + lane, tn);
+            fetch(jobs[jn], pb ^ 1, wv * 64 + lane, tn);
             total = tot_cur;
         }
 
         // ---- x FFT: accumulator -> padded tile -> ... -> kept frequencies
+        const int N1 = g.N1, h1 = N1 / 2;
         auto load_acc = [&](int row, int i) -> C {
             const int a = row * RS + i;
             if (FIX) {
@@ -470,12 +490,13 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
             if (f < N1 - h1) kk = f + h1;
             else if (f >= n1 - h1) kk = f - (n1 - h1);
             else return;
-            v = cscale(v, phinv1[kk]);
-            if (final_conj) v.y = -v.y;
-            const int rz = (int)fdiv((uint32_t)row, dby), ry = row - rz * by;
-            G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
+            v = cscale(v, J.phinv1[kk]);
+            if (J.final_conj) v.y = -v.y;
+            const int rz = (int)fdiv((uint32_t)row, J.dby), ry = row - rz * by;
+            J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
         };
         {
+            const RadixPlan &rp1 = J.rp1;
             int Ns = 1;
             for (int s = 0; s < rp1.nst; ++s) {
                 const int Rr = rp1.r[s];
@@ -497,8 +518,11 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(Geo g, RadixPlan rp1, con
             }
         }
         if (!has_next) break;
-        __syncthreads();  // every read of the tile done before it is zeroed
-        seq.cur = next;
+        __syncthreads();  // every read of the tile (and the twiddles) done
+        if (jn != j) load_tw(jobs[jn]);
+        cur = nxt;
+        pb ^= 1;
+        j = jn;
         t = tn;
         y0 = y0n;
         z0 = z0n;
@@ -511,19 +535,19 @@ int spread_max_runs() { return MAX_RUNS; }
 int spread_max_tile_b() { return MAX_TILE_B; }
 template <typename R> size_t spread_lds(int rows, int n1, int w) { return spread_lds_bytes<R>(rows, n1, w); }
 
+__global__ void k_zero_i32(int *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 template <typename R, int W>
-static void launch_w(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
-                     const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
-                     const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
-                     const float *amax, hipStream_t st)
+static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, const int *xr, int *wc, size_t lds,
+                     hipStream_t st)
 {
     constexpr int E = SpreadCfg<R>::E, NT = SpreadCfg<R>::NT;
-    const int by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
-    const int nty = (g.dim >= 2) ? g.n2 / by : 1;
-    const int ntiles = nty * ((g.dim >= 3) ? g.n3 / bz : 1);
-    const size_t lds = spread_lds_bytes<R>(by * bz, g.n1, W);
     const void *fn = reinterpret_cast<const void *>(&k_spread_xfft<R, E, NT, W>);
-    static int resident = 0;  // workgroups per CU at this kernel's LDS size
+    static int resident = 0;  // workgroups per CU at this LDS size
     static size_t resident_lds = 0;
     static int ncu = 0;
     if (resident == 0 || resident_lds != lds) {
@@ -535,28 +559,20 @@ static void launch_w(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R>
         resident = nb;
         resident_lds = lds;
     }
-    const int ntot = ntiles * ntr;
     int grid = std::max(1, ncu) * resident;
     grid = ((grid + 7) / 8) * 8;
-    grid = std::min(grid, ((ntot + 7) / 8) * 8);
-    hipLaunchKernelGGL((k_spread_xfft<R, E, NT, W>), dim3(grid), dim3(NT), lds, st, g, rp1, srec, cs, c_tstride,
-                       binstart, bin_tstride, tw1, phinv1, G1, final_conj, ntr, ntot, make_fdiv((uint32_t)ntr),
-                       make_fdiv((uint32_t)nty), make_fdiv((uint32_t)by), amax);
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, wc, 8);
+    hipLaunchKernelGGL((k_spread_xfft<R, E, NT, W>), dim3(grid), dim3(NT), lds, st, jobs, tile0s, njobs, xr, wc);
 }
 
 template <typename R>
-void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec<R> *srec,
-                        const typename cplx<R>::T *cs, int64_t c_tstride, const int64_t *binstart, int bin_tstride,
-                        const typename cplx<R>::T *tw1, const R *phinv1, typename cplx<R>::T *G1, int final_conj,
-                        const float *amax, hipStream_t st)
+void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
+                        size_t lds, hipStream_t st)
 {
-    if (ntr <= 0) return;
+    if (njobs <= 0) return;
 #define MPSQ_SPREAD_W(WV)                                                                                             \
-    case WV:                                                                                                          \
-        launch_w<R, WV>(g, rp1, ntr, srec, cs, c_tstride, binstart, bin_tstride, tw1, phinv1, G1, final_conj, amax,  \
-                        st);                                                                                          \
-        break;
-    switch (g.w) {
+    case WV: launch_w<R, WV>(jobs, tile0s, njobs, xr, wc, lds, st); break;
+    switch (w) {
         MPSQ_SPREAD_W(2)
         MPSQ_SPREAD_W(3)
         MPSQ_SPREAD_W(4)
@@ -580,9 +596,8 @@ void launch_spread_xfft(const Geo &g, const RadixPlan &rp1, int ntr, const PtRec
 #define MPSQ_SPREAD_INST(R)                                                                                           \
     template int spread_tile_elems<R>();                                                                              \
     template size_t spread_lds<R>(int, int, int);                                                                     \
-    template void launch_spread_xfft<R>(const Geo &, const RadixPlan &, int, const PtRec<R> *, const cplx<R>::T *,   \
-                                        int64_t, const int64_t *, int, const cplx<R>::T *, const R *, cplx<R>::T *,   \
-                                        int, const float *, hipStream_t);
+    template void launch_spread_jobs<R>(const SpreadJob<R> *, const int *, int, int, const int *, int *, size_t,     \
+                                        hipStream_t);
 
 MPSQ_SPREAD_INST(float)
 MPSQ_SPREAD_INST(double)
