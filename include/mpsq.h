@@ -195,6 +195,39 @@ int mpsq_sqw_get_timing(mpsq_sqw eng, double *ms, int64_t *launches, int n);
 int64_t mpsq_sqw_nq(mpsq_sqw eng);
 int mpsq_sqw_nact(mpsq_sqw eng);
 
+/* ---------------- batches of S(Q,w) jobs ----------------
+ * No counterpart in the reference, which computes one map per dialogue
+ * session.  A batch runs many uneven jobs (different supercells, species,
+ * frame counts and mode boxes) through one launch per phase: staging, bin
+ * sort, one persistent spread + x-FFT launch over every tile of every job
+ * (one per kernel width present), the column passes; the per-job reductions
+ * follow on the same stream.  Every spec must have combine = 1 and the same
+ * precision; specs[0].opts gives the device and the stream. */
+typedef struct mpsq_batch_s *mpsq_batch;
+int mpsq_batch_create(int njobs, const mpsq_sqw_spec *specs, mpsq_batch *batch);
+/* d_pos[j]: device float[T[j]][n_tot_j][4]; d_amp[j]: device complex [T[j]][NQ_j];
+ * r: njobs reduction specs or NULL (amplitudes only); d_S[j], d_Stot[j] as in
+ * mpsq_sqw_reduce (d_S / d_Stot may be NULL).  use_graph = 1: the second call
+ * with the same arguments captures the whole sequence in a hipGraph, later
+ * calls replay it.  Returns after the stream has drained. */
+int mpsq_batch_run(mpsq_batch b, const float *const *d_pos, const int64_t *T, void *const *d_amp,
+                   const mpsq_reduce_spec *r, double *const *d_S, double *const *d_Stot, int use_graph);
+/* Device bytes of intermediates per launch round (default 16 GiB): a batch
+ * larger than this runs in several rounds. */
+int mpsq_batch_set_budget(mpsq_batch b, int64_t bytes);
+int mpsq_batch_set_timing(mpsq_batch b, int on);
+int mpsq_batch_get_timing(mpsq_batch b, double *ms, int64_t *launches, int n);
+int mpsq_batch_destroy(mpsq_batch b);
+
+/* ---------------- spread tile autotuner ----------------
+ * No counterpart in the reference.  Times every spread tile shape (by, bz)
+ * that fits the spec's geometry on the first frames of d_pos (at most 4) and
+ * returns the fastest with its time per frame.  The choice is cached for the
+ * process per (dim, fine grid, kernel width, precision); engines and batches
+ * created afterwards with opts.tile_by = tile_bz = 0 use it. */
+int mpsq_autotune_tiles(const mpsq_sqw_spec *spec, const float *d_pos, int64_t T, int *tile_by, int *tile_bz,
+                        double *ms_per_frame);
+
 /* Library version string. */
 const char *mpsq_version(void);
 

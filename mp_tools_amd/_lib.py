@@ -36,6 +36,8 @@ EXPORTS = [
     "mpsq_sqw_default_spec", "mpsq_sqw_create", "mpsq_sqw_amplitudes", "mpsq_sqw_reduce",
     "mpsq_sqw_destroy", "mpsq_sqw_sync", "mpsq_sqw_nq", "mpsq_sqw_nact", "mpsq_version",
     "mpsq_sqw_set_timing", "mpsq_sqw_get_timing",
+    "mpsq_batch_create", "mpsq_batch_run", "mpsq_batch_set_budget", "mpsq_batch_set_timing",
+    "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
 ]
 STAGES = ("stage", "sort", "spread_xfft", "colfft", "final_colfft", "reduce")
 
@@ -128,6 +130,13 @@ def load():
     L.mpsq_version.restype = C.c_char_p
     L.mpsq_sqw_set_timing.argtypes = [vp, i32]
     L.mpsq_sqw_get_timing.argtypes = [vp, vp, vp, i32]
+    L.mpsq_batch_create.argtypes = [i32, vp, vp]
+    L.mpsq_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32]
+    L.mpsq_batch_set_budget.argtypes = [vp, i64]
+    L.mpsq_batch_set_timing.argtypes = [vp, i32]
+    L.mpsq_batch_get_timing.argtypes = [vp, vp, vp, i32]
+    L.mpsq_batch_destroy.argtypes = [vp]
+    L.mpsq_autotune_tiles.argtypes = [vp, vp, i64, vp, vp, vp]
     _lib = L
     return L
 

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <vector>
@@ -355,6 +356,19 @@ static std::vector<int> xcd_ranges(const std::vector<double> &job_cost, const st
     return xr;
 }
 
+// Spread tile shapes chosen by the autotuner, per (dim, n1, n2, n3, w, fp64).
+struct TuneKey {
+    int v[6];
+    bool operator<(const TuneKey &o) const
+    {
+        for (int i = 0; i < 6; ++i)
+            if (v[i] != o.v[i]) return v[i] < o.v[i];
+        return false;
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::pair<int, int>> g_tuned;
+
 // ---------------------------------------------------------------------------
 // geometry shared by plans and the S(Q,w) engine
 // ---------------------------------------------------------------------------
@@ -394,7 +408,7 @@ template <typename R> struct Core {
             st = (hipStream_t)o->stream;
             own_stream = false;
         } else {
-            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreate(&st));  // blocking: ordered with the legacy default stream
             own_stream = true;
         }
         int w;
@@ -437,6 +451,15 @@ template <typename R> struct Core {
                         bz = b2;
                     }
                 }
+        }
+        if (o->tile_by <= 0 && o->tile_bz <= 0) {
+            const TuneKey key{{dim, (int)n[0], (int)n[1], (int)n[2], w, sizeof(R) == 8 ? 1 : 0}};
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) {
+                by = it->second.first;
+                bz = it->second.second;
+            }
         }
         if (o->tile_by > 0 && dim >= 2 && n[1] % o->tile_by == 0) by = o->tile_by;
         if (o->tile_bz > 0 && dim >= 3 && n[2] % o->tile_bz == 0) bz = o->tile_bz;
@@ -568,7 +591,7 @@ template <typename R> struct Core {
         J.ntr = ntr;
         J.ntiles = ntiles();
         J.tile0 = 0;
-        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        const int by = (dim >= 2) ? g.by : 1;
         J.dntr = make_fdiv((uint32_t)ntr);
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
@@ -865,7 +888,7 @@ template <typename R> struct Part {
 };
 
 template <typename R>
-static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipStream_t st, StageTimer *tm);
+static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena &A, hipStream_t st, StageTimer *tm);
 
 template <typename R> struct Engine {
     Core<R> core;
@@ -981,7 +1004,7 @@ template <typename R> struct Engine {
             parts[0].b = b;
             parts[0].pos = d_pos + (size_t)f0 * n_tot * 4;
             parts[0].amp = static_cast<Cx<R> *>(d_amp) + (size_t)f0 * amp_frame_elems();
-            run_parts<R>(parts, ws, c.st, &tm);
+            run_parts<R>(parts, ws, ws.tab, c.st, &tm);
         }
         HIPCHK(hipGetLastError());
         int hbad = 0;
@@ -992,6 +1015,15 @@ template <typename R> struct Engine {
     }
 
     void reduce(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot)
+    {
+        reduce_on(d_amp, T, r, d_S, d_Stot, core.st, true);
+    }
+
+    // Enqueue the reduction on stream st (sync: wait and collect timings).
+    // The window and time-FFT twiddles are uploaded when T or the window
+    // kind change, so a repeated call enqueues kernels only.
+    void reduce_on(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot,
+                   hipStream_t st, bool sync)
     {
         Core<R> &c = core;
         if (!combine || !d_amp || !r || (!d_S && r->n_f > 0) || T < 1) throw Err{MPSQ_ERR_BAD_ARG};
@@ -1021,27 +1053,28 @@ template <typename R> struct Engine {
                     else win_h[i] = 0.5 * (1.0 - std::cos(2.0 * M_PI * (double)(i + 1) / (double)(T + 1)));
                 }
                 win_d.ensure(sizeof(double) * (size_t)T);
-                HIPCHK(hipMemcpyAsync(win_d.p, win_h.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, c.st));
+                HIPCHK(hipMemcpyAsync(win_d.p, win_h.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, st));
                 win_n = (int)T;
                 win_kind = r->win_kind;
             }
             if (twT_n != (int)T) {
-                upload_twiddles<R>(twT, (int)T, c.st, twT_h);
+                upload_twiddles<R>(twT, (int)T, st, twT_h);
                 twT_n = (int)T;
                 rpT = rp;
             }
             int cols = 1;
             while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= 64) cols *= 2;
-            const int ev = tm.begin(ST_REDUCE, c.st);
-            launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, c.st);
-            tm.end(ev, c.st);
+            const int ev = tm.begin(ST_REDUCE, st);
+            launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, st);
+            tm.end(ev, st);
         } else {
-            const int ev = tm.begin(ST_REDUCE, c.st);
-            launch_sqw_sliding<R>(p, amp, r->d_qfac, d_S, d_Stot, c.st);
-            tm.end(ev, c.st);
+            const int ev = tm.begin(ST_REDUCE, st);
+            launch_sqw_sliding<R>(p, amp, r->d_qfac, d_S, d_Stot, st);
+            tm.end(ev, st);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(c.st));
+        if (!sync) return;
+        HIPCHK(hipStreamSynchronize(st));
         tm.collect();
     }
 };
@@ -1055,7 +1088,7 @@ template <typename R> struct Engine {
 //   columns  y pass over all transforms, final pass (species combine)
 // ---------------------------------------------------------------------------
 template <typename R>
-static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipStream_t st, StageTimer *tm)
+static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena &A, hipStream_t st, StageTimer *tm)
 {
     StageTimer dummy;
     if (!tm) tm = &dummy;
@@ -1220,7 +1253,6 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipSt
         }
     }
     // ---- tables
-    Arena &A = ws.tab;
     A.begin();
     const size_t o_stj = A.put(stj), o_stb = A.put(stb), o_ch = A.put(chunks), o_ti = A.put(tinfo);
     for (size_t wi = 0; wi < sl.size(); ++wi) {
@@ -1261,9 +1293,236 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, hipSt
     tm->end(ev, st);
 }
 
+// ---------------------------------------------------------------------------
+// Batch of S(Q,w) jobs (uneven geometries, frame counts, species): all jobs'
+// frames go through one launch per phase (memory permitting: rounds cut at a
+// device-memory budget), the reductions follow on the same stream, and the
+// whole sequence can be captured once in a hipGraph and replayed.
+// ---------------------------------------------------------------------------
+template <typename R> struct Batch {
+    std::vector<Engine<R> *> jobs;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    int device = 0;
+    size_t budget = (size_t)16 << 30;  // intermediates per round
+    Workspace<R> ws;
+    std::vector<Arena *> arenas;  // job tables of each round (all alive in a captured graph)
+    StageTimer tm;
+    // captured sequence
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void *> gkey;  // pointers / sizes the graph was captured with
+
+    ~Batch()
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (Engine<R> *e : jobs) delete e;
+        for (Arena *a : arenas) delete a;
+        if (own_stream && st) (void)hipStreamDestroy(st);
+    }
+
+    void create(int njobs, const mpsq_sqw_spec *specs)
+    {
+        if (njobs < 1 || njobs > MAX_JOBS || !specs) throw Err{MPSQ_ERR_BAD_ARG};
+        device = specs[0].opts.device;
+        HIPCHK(hipSetDevice(device));
+        if (specs[0].opts.stream) {
+            st = (hipStream_t)specs[0].opts.stream;
+        } else {
+            HIPCHK(hipStreamCreate(&st));  // blocking: ordered with the legacy default stream
+            own_stream = true;
+        }
+        for (int j = 0; j < njobs; ++j) {
+            if (specs[j].precision != (sizeof(R) == 4 ? MPSQ_FP32 : MPSQ_FP64) || !specs[j].combine)
+                throw Err{MPSQ_ERR_BAD_ARG};
+            mpsq_sqw_spec sj = specs[j];
+            sj.opts.device = device;
+            sj.opts.stream = st;
+            Engine<R> *e = new Engine<R>();
+            jobs.push_back(e);
+            e->create(&sj);
+        }
+    }
+
+    // frames of every job in rounds of at most `budget` bytes of intermediates
+    std::vector<std::vector<Part<R>>> rounds(const float *const *d_pos, const int64_t *T, void *const *d_amp) const
+    {
+        std::vector<std::vector<Part<R>>> out(1);
+        size_t used = 0;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const Engine<R> &e = *jobs[j];
+            const size_t per = std::max<size_t>(e.bytes_per_frame(), 1);
+            int64_t f = 0;
+            while (f < T[j]) {
+                int64_t room = (int64_t)((budget > used ? budget - used : 0) / per);
+                if (room < 1 && !out.back().empty()) {
+                    out.emplace_back();
+                    used = 0;
+                    continue;
+                }
+                const int b = (int)std::max<int64_t>(1, std::min<int64_t>(room, T[j] - f));
+                Part<R> p;
+                p.e = jobs[j];
+                p.f0 = f;
+                p.b = b;
+                p.pos = d_pos[j] + (size_t)f * e.n_tot * 4;
+                p.amp = static_cast<Cx<R> *>(d_amp[j]) + (size_t)f * e.amp_frame_elems();
+                out.back().push_back(p);
+                used += per * (size_t)b;
+                f += b;
+            }
+        }
+        return out;
+    }
+
+    void enqueue(const float *const *d_pos, const int64_t *T, void *const *d_amp, const mpsq_reduce_spec *r,
+                 double *const *d_S, double *const *d_Stot, StageTimer *timer)
+    {
+        ws.bad.ensure(sizeof(int));
+        launch_zero_i32(ws.bad.template as<int>(), 1, st);
+        const std::vector<std::vector<Part<R>>> rs = rounds(d_pos, T, d_amp);
+        while (arenas.size() < rs.size()) arenas.push_back(new Arena());
+        for (size_t k = 0; k < rs.size(); ++k) run_parts<R>(rs[k], ws, *arenas[k], st, timer);
+        if (r)
+            for (size_t j = 0; j < jobs.size(); ++j)
+                jobs[j]->reduce_on(d_amp[j], T[j], &r[j], d_S ? d_S[j] : nullptr, d_Stot ? d_Stot[j] : nullptr, st,
+                                   false);
+    }
+
+    void check_bad()
+    {
+        int hbad = 0;
+        HIPCHK(hipMemcpyAsync(&hbad, ws.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
+    }
+
+    // use_graph: the first call with a given set of pointers runs eagerly
+    // (buffers and tables are allocated), the second captures the sequence,
+    // later calls replay it.
+    void run(const float *const *d_pos, const int64_t *T, void *const *d_amp, const mpsq_reduce_spec *r,
+             double *const *d_S, double *const *d_Stot, bool use_graph)
+    {
+        HIPCHK(hipSetDevice(device));
+        std::vector<const void *> key;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            key.push_back(d_pos[j]);
+            key.push_back((const void *)(intptr_t)T[j]);
+            key.push_back(d_amp[j]);
+            key.push_back(d_S ? d_S[j] : nullptr);
+            key.push_back(d_Stot ? d_Stot[j] : nullptr);
+            if (r) {
+                key.push_back((const void *)(intptr_t)r[j].n_int);
+                key.push_back((const void *)(intptr_t)r[j].n_f);
+                key.push_back((const void *)(intptr_t)r[j].m_lo);
+                key.push_back(r[j].d_qfac);
+            }
+        }
+        if (use_graph && gexec && key == gkey) {
+            HIPCHK(hipGraphLaunch(gexec, st));
+            check_bad();
+            return;
+        }
+        if (use_graph && key == gkey && !gexec) {
+            // capture: everything is allocated and the tables are current
+            HIPCHK(hipStreamSynchronize(st));
+            for (Arena *a : arenas) a->begin();  // nothing pending may be waited on inside the capture
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            enqueue(d_pos, T, d_amp, r, d_S, d_Stot, nullptr);
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamEndCapture(st, &g));
+            graph = g;
+            HIPCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            HIPCHK(hipGraphLaunch(gexec, st));
+            check_bad();
+            return;
+        }
+        if (gexec) {
+            (void)hipGraphExecDestroy(gexec);
+            gexec = nullptr;
+        }
+        if (graph) {
+            (void)hipGraphDestroy(graph);
+            graph = nullptr;
+        }
+        gkey = key;
+        enqueue(d_pos, T, d_amp, r, d_S, d_Stot, &tm);
+        HIPCHK(hipGetLastError());
+        check_bad();
+        tm.collect();
+    }
+};
+
+// ---- spread tile autotuner ----
+template <typename R>
+static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t T, int *tby, int *tbz, double *ms)
+{
+    const int64_t Tt = std::min<int64_t>(T, 4);
+    // the geometry (fine grid, width) of the heuristic choice
+    Engine<R> probe;
+    probe.create(spec);
+    const Core<R> &c = probe.core;
+    const int dim = c.dim, w = c.g.w, n1 = c.g.n1, n2 = c.g.n2, n3 = c.g.n3;
+    const int TE = spread_tile_elems<R>(), bmax = spread_max_tile_b();
+    DevBuf amp;
+    amp.ensure(sizeof(Cx<R>) * (size_t)Tt * (size_t)probe.amp_frame_elems());
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    int bb[2] = {c.g.by, c.g.bz};
+    for (int by = 1; by <= (dim >= 2 ? bmax : 1); by *= 2)
+        for (int bz = 1; bz <= (dim >= 3 ? bmax : 1); bz *= 2) {
+            if ((int64_t)by * bz * n1 > TE) continue;
+            if (dim >= 2 && (n2 % by || by + w > n2)) continue;
+            if (dim >= 3 && (n3 % bz || bz + w > n3)) continue;
+            if (spread_lds<R>(by * bz, n1, w) > 156 * 1024) continue;
+            mpsq_sqw_spec sj = *spec;
+            sj.opts.tile_by = by;
+            sj.opts.tile_bz = bz;
+            Engine<R> e;
+            e.create(&sj);
+            e.amplitudes(d_pos, Tt, amp.p);  // warm-up: buffers, tables
+            HIPCHK(hipEventRecord(e0, e.core.st));
+            e.amplitudes(d_pos, Tt, amp.p);
+            HIPCHK(hipEventRecord(e1, e.core.st));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (t < best) {
+                best = t;
+                bb[0] = by;
+                bb[1] = bz;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    {
+        const TuneKey key{{dim, n1, n2, n3, w, sizeof(R) == 8 ? 1 : 0}};
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[key] = std::make_pair(bb[0], bb[1]);
+    }
+    if (tby) *tby = bb[0];
+    if (tbz) *tbz = bb[1];
+    if (ms) *ms = best / (double)Tt;
+    return 0;
+}
+
 }  // namespace mpsq
 
 using namespace mpsq;
+
+struct mpsq_batch_s {
+    Batch<float> *bf = nullptr;
+    Batch<double> *bd = nullptr;
+    ~mpsq_batch_s()
+    {
+        delete bf;
+        delete bd;
+    }
+};
 
 struct mpsq_sqw_s {
     int precision = MPSQ_FP32;
@@ -1585,6 +1844,108 @@ int mpsq_sqw_destroy(mpsq_sqw e)
 {
     delete e;
     return 0;
+}
+
+// ---- batches ----
+int mpsq_batch_create(int njobs, const mpsq_sqw_spec *specs, mpsq_batch *batch)
+{
+    if (!batch) return MPSQ_ERR_BAD_ARG;
+    *batch = nullptr;
+    if (njobs < 1 || !specs) return MPSQ_ERR_BAD_ARG;
+    mpsq_batch_s *b = new (std::nothrow) mpsq_batch_s();
+    if (!b) return MPSQ_ERR_DEVICE;
+    try {
+        if (specs[0].precision == MPSQ_FP64) {
+            b->bd = new Batch<double>();
+            b->bd->create(njobs, specs);
+        } else {
+            b->bf = new Batch<float>();
+            b->bf->create(njobs, specs);
+        }
+    } catch (const Err &er) {
+        delete b;
+        return er.code;
+    } catch (...) {
+        delete b;
+        return MPSQ_ERR_DEVICE;
+    }
+    *batch = b;
+    int warn = 0;
+    if (b->bd)
+        for (auto *e : b->bd->jobs) warn = std::max(warn, e->core.warn);
+    if (b->bf)
+        for (auto *e : b->bf->jobs) warn = std::max(warn, e->core.warn);
+    return warn;
+}
+
+int mpsq_batch_run(mpsq_batch b, const float *const *d_pos, const int64_t *T, void *const *d_amp,
+                   const mpsq_reduce_spec *r, double *const *d_S, double *const *d_Stot, int use_graph)
+{
+    if (!b || !d_pos || !T || !d_amp) return MPSQ_ERR_BAD_ARG;
+    try {
+        const size_t nj = b->bf ? b->bf->jobs.size() : b->bd->jobs.size();
+        for (size_t j = 0; j < nj; ++j) {
+            if (!d_pos[j] || !d_amp[j] || T[j] < 1) throw Err{MPSQ_ERR_BAD_ARG};
+            if (r && (r[j].n_int < 1 || r[j].n_int > T[j] || r[j].n_f < 0 || !(r[j].nat > 0) ||
+                      (r[j].n_f > 0 && (!d_S || !d_S[j]))))
+                throw Err{MPSQ_ERR_BAD_ARG};
+        }
+        if (b->bf) b->bf->run(d_pos, T, d_amp, r, d_S, d_Stot, use_graph != 0);
+        else b->bd->run(d_pos, T, d_amp, r, d_S, d_Stot, use_graph != 0);
+    } catch (const Err &er) {
+        return er.code;
+    } catch (...) {
+        return MPSQ_ERR_DEVICE;
+    }
+    return 0;
+}
+
+int mpsq_batch_set_budget(mpsq_batch b, int64_t bytes)
+{
+    if (!b || bytes < 1) return MPSQ_ERR_BAD_ARG;
+    if (b->bf) b->bf->budget = (size_t)bytes;
+    else b->bd->budget = (size_t)bytes;
+    return 0;
+}
+
+int mpsq_batch_set_timing(mpsq_batch b, int on)
+{
+    if (!b) return MPSQ_ERR_BAD_ARG;
+    StageTimer &tm = b->bf ? b->bf->tm : b->bd->tm;
+    tm.on = on != 0;
+    tm.reset();
+    return 0;
+}
+
+int mpsq_batch_get_timing(mpsq_batch b, double *ms, int64_t *launches, int n)
+{
+    if (!b || n < 0) return MPSQ_ERR_BAD_ARG;
+    const StageTimer &tm = b->bf ? b->bf->tm : b->bd->tm;
+    for (int k = 0; k < n && k < StageTimer::K; ++k) {
+        if (ms) ms[k] = tm.ms[k];
+        if (launches) launches[k] = tm.n[k];
+    }
+    return 0;
+}
+
+int mpsq_batch_destroy(mpsq_batch b)
+{
+    delete b;
+    return 0;
+}
+
+int mpsq_autotune_tiles(const mpsq_sqw_spec *spec, const float *d_pos, int64_t T, int *tile_by, int *tile_bz,
+                        double *ms_per_frame)
+{
+    if (!spec || !d_pos || T < 1) return MPSQ_ERR_BAD_ARG;
+    try {
+        if (spec->precision == MPSQ_FP64) return autotune_impl<double>(spec, d_pos, T, tile_by, tile_bz, ms_per_frame);
+        return autotune_impl<float>(spec, d_pos, T, tile_by, tile_bz, ms_per_frame);
+    } catch (const Err &er) {
+        return er.code;
+    } catch (...) {
+        return MPSQ_ERR_DEVICE;
+    }
 }
 
 }  // extern "C"

@@ -172,6 +172,8 @@ template <typename R> int colfft_tile_elems(bool final);
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
                         size_t lds, hipStream_t st);
+// p[0..n) = 0 by a kernel (graph-capturable, unlike a memset node)
+void launch_zero_i32(int *p, int n, hipStream_t st);
 // amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)
 void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t st);
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final);

@@ -541,6 +541,11 @@ __global__ void k_zero_i32(int *__restrict__ p, int n)
     if (i < n) p[i] = 0;
 }
 
+void launch_zero_i32(int *p, int n, hipStream_t st)
+{
+    if (n > 0) hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+}
+
 template <typename R, int W>
 static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, const int *xr, int *wc, size_t lds,
                      hipStream_t st)

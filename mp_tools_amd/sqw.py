@@ -20,6 +20,82 @@ import numpy as np
 from . import _lib
 
 
+def make_spec(dim, n_modes, species_counts, eps=1e-6, precision="fp32", iflag=-1,
+              weights=None, weight_table=None, mask=None, combine=True,
+              centre=(0.0, 0.0, 0.0), axes=None, tpe=(1.0, 1.0, 1.0), q0=(0.0, 0.0, 0.0),
+              window=False, vacancy_check=True, select_mode=0, n_row=(1, 1, 1),
+              pmin=(0.0, 0.0, 0.0), pmax=(0.0, 0.0, 0.0), margin=0.1, at_ind=None,
+              batch_frames=0, device=0, stream="torch", tile_by=0, tile_bz=0):
+    """An mpsq_sqw_spec from the reference's inputs (see SqwEngine); returns
+    (spec, host arrays the spec points into, derived sizes)."""
+    import torch
+
+    L = _lib.load()
+    dim = int(dim)
+    n_modes = tuple(int(v) for v in n_modes)
+    if len(n_modes) != dim:
+        raise ValueError("n_modes must have dim entries")
+    NQ = int(np.prod(n_modes))
+    fp64 = precision == "fp64"
+    counts = np.ascontiguousarray(species_counts, np.int64)
+    nsp = counts.size
+    n_tot = int(counts.sum())
+    s = _lib.SqwSpec()
+    L.mpsq_sqw_default_spec(C.byref(s))
+    s.dim = dim
+    for d in range(3):
+        s.n_modes[d] = n_modes[d] if d < dim else 1
+    s.eps = float(eps)
+    s.precision = _lib.FP64 if fp64 else _lib.FP32
+    s.iflag = int(iflag)
+    s.n_tot = n_tot
+    s.nsp = nsp
+    keep = [counts]
+    s.sp_count = counts.ctypes.data
+    if mask is not None:
+        m = np.ascontiguousarray(mask, np.int32)
+        keep.append(m)
+        s.mask = m.ctypes.data
+    if weights is not None:
+        w = np.ascontiguousarray(weights, np.float64)
+        keep.append(w)
+        s.w_scalar = w.ctypes.data
+    if weight_table is not None:
+        wt = np.ascontiguousarray(weight_table, np.float32).reshape(nsp, NQ)
+        keep.append(wt)
+        s.w_table = wt.ctypes.data
+    s.combine = 1 if combine else 0
+    ax = np.eye(3) if axes is None else np.asarray(axes, np.float64).reshape(3, 3)
+    for d in range(3):
+        s.centre[d] = float(centre[d])
+        s.tpe[d] = float(tpe[d])
+        s.q0[d] = float(q0[d])
+        s.n_row[d] = int(n_row[d])
+        s.pmin[d] = float(pmin[d])
+        s.pmax[d] = float(pmax[d])
+    for k in range(9):
+        s.axes[k] = float(ax.ravel()[k])
+    s.window = int(bool(window))
+    s.vacancy_check = int(bool(vacancy_check))
+    s.select_mode = int(select_mode)
+    s.margin = float(margin)
+    if at_ind is not None:
+        ai = np.ascontiguousarray(at_ind, np.int32).reshape(n_tot, 4)
+        keep.append(ai)
+        s.at_ind = ai.ctypes.data
+    s.batch_frames = int(batch_frames)
+    s.opts.device = int(device)
+    if stream == "torch":
+        torch.cuda.set_device(int(device))
+        stream = torch.cuda.current_stream().cuda_stream
+    s.opts.stream = stream
+    s.opts.tile_by, s.opts.tile_bz = int(tile_by), int(tile_bz)
+    info = dict(dim=dim, n_modes=n_modes, NQ=NQ, precision=precision, fp64=fp64,
+                cdtype=torch.complex128 if fp64 else torch.complex64, nsp=nsp, n_tot=n_tot,
+                combine=bool(combine))
+    return s, keep, info
+
+
 class SqwEngine:
     """Per-species NUFFT + species combine + time reduction on one GPU.
 
@@ -32,77 +108,11 @@ class SqwEngine:
     ``q0`` the Q-centre of the strengths' phase exp(-i 2 pi q0 . r).
     """
 
-    def __init__(self, dim, n_modes, species_counts, eps=1e-6, precision="fp32", iflag=-1,
-                 weights=None, weight_table=None, mask=None, combine=True,
-                 centre=(0.0, 0.0, 0.0), axes=None, tpe=(1.0, 1.0, 1.0), q0=(0.0, 0.0, 0.0),
-                 window=False, vacancy_check=True, select_mode=0, n_row=(1, 1, 1),
-                 pmin=(0.0, 0.0, 0.0), pmax=(0.0, 0.0, 0.0), margin=0.1, at_ind=None,
-                 batch_frames=0, device=0, stream="torch", tile_by=0, tile_bz=0):
-        import torch
-
+    def __init__(self, dim, n_modes, species_counts, **kw):
         self._L = _lib.load()
-        self.dim = int(dim)
-        self.n_modes = tuple(int(v) for v in n_modes)
-        if len(self.n_modes) != self.dim:
-            raise ValueError("n_modes must have dim entries")
-        self.NQ = int(np.prod(self.n_modes))
-        self.precision = precision
-        self.fp64 = precision == "fp64"
-        self.cdtype = torch.complex128 if self.fp64 else torch.complex64
-        counts = np.ascontiguousarray(species_counts, np.int64)
-        self.nsp = counts.size
-        self.n_tot = int(counts.sum())
-        s = _lib.SqwSpec()
-        self._L.mpsq_sqw_default_spec(C.byref(s))
-        s.dim = self.dim
-        for d in range(3):
-            s.n_modes[d] = self.n_modes[d] if d < self.dim else 1
-        s.eps = float(eps)
-        s.precision = _lib.FP64 if self.fp64 else _lib.FP32
-        s.iflag = int(iflag)
-        s.n_tot = self.n_tot
-        s.nsp = self.nsp
-        self._keep = [counts]
-        s.sp_count = counts.ctypes.data
-        if mask is not None:
-            m = np.ascontiguousarray(mask, np.int32)
-            self._keep.append(m)
-            s.mask = m.ctypes.data
-        if weights is not None:
-            w = np.ascontiguousarray(weights, np.float64)
-            self._keep.append(w)
-            s.w_scalar = w.ctypes.data
-        if weight_table is not None:
-            wt = np.ascontiguousarray(weight_table, np.float32).reshape(self.nsp, self.NQ)
-            self._keep.append(wt)
-            s.w_table = wt.ctypes.data
-        s.combine = 1 if combine else 0
-        self.combine = bool(combine)
-        ax = np.eye(3) if axes is None else np.asarray(axes, np.float64).reshape(3, 3)
-        for d in range(3):
-            s.centre[d] = float(centre[d])
-            s.tpe[d] = float(tpe[d])
-            s.q0[d] = float(q0[d])
-            s.n_row[d] = int(n_row[d])
-            s.pmin[d] = float(pmin[d])
-            s.pmax[d] = float(pmax[d])
-        for k in range(9):
-            s.axes[k] = float(ax.ravel()[k])
-        s.window = int(bool(window))
-        s.vacancy_check = int(bool(vacancy_check))
-        s.select_mode = int(select_mode)
-        s.margin = float(margin)
-        if at_ind is not None:
-            ai = np.ascontiguousarray(at_ind, np.int32).reshape(self.n_tot, 4)
-            self._keep.append(ai)
-            s.at_ind = ai.ctypes.data
-        s.batch_frames = int(batch_frames)
-        s.opts.device = int(device)
-        if stream == "torch":
-            torch.cuda.set_device(int(device))
-            stream = torch.cuda.current_stream().cuda_stream
-        s.opts.stream = stream
-        s.opts.tile_by, s.opts.tile_bz = int(tile_by), int(tile_bz)
+        s, keep, info = make_spec(dim, n_modes, species_counts, **kw)
+        self._keep = keep
+        self.__dict__.update(info)
         h = C.c_void_p()
         rc = self._L.mpsq_sqw_create(C.byref(s), C.byref(h))
         _lib.check("sqw_create", rc)
@@ -172,3 +182,115 @@ class SqwEngine:
             self.destroy()
         except Exception:
             pass
+
+
+def _reduce_spec(n_int, win_kind=0, dt=1.0, nat=1.0, m_lo=0, n_f=None, subtract_mean=True, qfac=None):
+    r = _lib.ReduceSpec()
+    r.n_int, r.win_kind, r.dt, r.nat = int(n_int), int(win_kind), float(dt), float(nat)
+    r.m_lo, r.n_f, r.subtract_mean = int(m_lo), int(n_int if n_f is None else n_f), int(bool(subtract_mean))
+    if qfac is not None:
+        r.d_qfac = qfac.data_ptr()
+    return r
+
+
+class SqwBatch:
+    """Many S(Q,w) jobs through one launch per phase (``mpsq_batch_*``).
+
+    ``jobs`` is a list of dicts of SqwEngine arguments (dim, n_modes,
+    species_counts, eps, weights, tpe, ...); all FP32 or all FP64, combined.
+    ``run`` takes one device trajectory per job and, optionally, one
+    reduction (dict of SqwEngine.reduce arguments) per job; with
+    ``use_graph`` the second call with the same tensors captures the whole
+    sequence in a hipGraph and later calls replay it.
+    """
+
+    def __init__(self, jobs, budget_bytes=None, stream="torch", device=0):
+        self._L = _lib.load()
+        specs = (_lib.SqwSpec * len(jobs))()
+        self._keep, self.info = [], []
+        for j, kw in enumerate(jobs):
+            kw = dict(kw)
+            kw.setdefault("stream", stream)
+            kw.setdefault("device", device)
+            s, keep, info = make_spec(kw.pop("dim"), kw.pop("n_modes"), kw.pop("species_counts"), **kw)
+            specs[j] = s
+            self._keep.append(keep)
+            self.info.append(info)
+        h = C.c_void_p()
+        rc = self._L.mpsq_batch_create(len(jobs), specs, C.byref(h))
+        _lib.check("batch_create", rc)
+        self.warning = rc
+        self._h = h
+        if budget_bytes:
+            _lib.check("batch_set_budget", self._L.mpsq_batch_set_budget(h, int(budget_bytes)))
+
+    def run(self, positions, reductions=None, out=None, use_graph=False):
+        """positions: list of CUDA float32 [T_j, n_tot_j, 4].  Returns the
+        amplitudes [T_j, NQ_j] and, with reductions, (S [n_f, NQ], S_tot [NQ])
+        per job.  Outputs are allocated on the first call and may be passed
+        back through ``out`` (required for graph replay)."""
+        import torch
+
+        nj = len(self.info)
+        if len(positions) != nj:
+            raise ValueError("one trajectory per job")
+        if out is None:
+            amps = [torch.empty((p.shape[0], self.info[j]["NQ"]), dtype=self.info[j]["cdtype"],
+                                device=p.device) for j, p in enumerate(positions)]
+            S = St = None
+            if reductions is not None:
+                S, St = [], []
+                for j, rd in enumerate(reductions):
+                    T = positions[j].shape[0]
+                    n_f = rd.get("n_f", rd.get("n_int", T))
+                    S.append(torch.empty((n_f, self.info[j]["NQ"]), dtype=torch.float64, device=positions[j].device))
+                    St.append(torch.empty(self.info[j]["NQ"], dtype=torch.float64, device=positions[j].device))
+            out = (amps, S, St)
+        amps, S, St = out
+        P = (C.c_void_p * nj)(*[p.data_ptr() for p in positions])
+        T = (C.c_int64 * nj)(*[p.shape[0] for p in positions])
+        A = (C.c_void_p * nj)(*[a.data_ptr() for a in amps])
+        R = dS = dSt = None
+        if reductions is not None:
+            R = (_lib.ReduceSpec * nj)()
+            for j, rd in enumerate(reductions):
+                rd = dict(rd)
+                rd.setdefault("n_int", positions[j].shape[0])
+                R[j] = _reduce_spec(**rd)
+            dS = (C.c_void_p * nj)(*[x.data_ptr() for x in S])
+            dSt = (C.c_void_p * nj)(*[x.data_ptr() for x in St])
+        rc = self._L.mpsq_batch_run(self._h, P, T, A, R, dS, dSt, int(bool(use_graph)))
+        _lib.check("batch_run", rc)
+        return out
+
+    def set_timing(self, on=True):
+        _lib.check("batch_set_timing", self._L.mpsq_batch_set_timing(self._h, int(bool(on))))
+
+    def timing(self):
+        ms = (C.c_double * 6)()
+        n = (C.c_int64 * 6)()
+        _lib.check("batch_get_timing", self._L.mpsq_batch_get_timing(self._h, ms, n, 6))
+        return {k: (ms[i], n[i]) for i, k in enumerate(_lib.STAGES)}
+
+    def destroy(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.mpsq_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def autotune_tiles(pos, dim, n_modes, species_counts, **kw):
+    """Time the spread tile shapes for this geometry on the first frames of
+    ``pos`` and cache the fastest for engines created afterwards.  Returns
+    (tile_by, tile_bz, ms per frame)."""
+    L = _lib.load()
+    s, keep, info = make_spec(dim, n_modes, species_counts, **kw)
+    by, bz, ms = C.c_int(), C.c_int(), C.c_double()
+    _lib.check("autotune_tiles", L.mpsq_autotune_tiles(C.byref(s), pos.data_ptr(), pos.shape[0],
+                                                      C.byref(by), C.byref(bz), C.byref(ms)))
+    return by.value, bz.value, ms.value

@@ -92,3 +92,128 @@ def phonon_trajectory(ncell: int, T: int, dt_ps: float = 0.01, n_phonons: int = 
         nz = torch.randn((R.shape[0], 3), generator=gen, dtype=torch.float64, device=dev)
         out[t, :, :3] = (Rt + u.T + nz * (noise / a)).to(torch.float32)
     return out, counts
+
+
+# ---------------------------------------------------------------------------
+# amorphous binary boxes and the batched-jobs workload
+# ---------------------------------------------------------------------------
+B_AL_U_FM = np.array([3.449, 8.417])   # Al, U coherent lengths (code/ref/neutron_xs.txt)
+
+
+def amorphous_box(side: float, density: float = 0.06, fractions=(0.92, 0.08), hard_core: float = 2.2,
+                  seed: int = 0, device="cpu", max_rounds: int = 2000):
+    """Random sequential addition of round(density * side^3) atoms, uniform in
+    a periodic cube [-side/2, side/2)^3 (Angstrom), rejecting any candidate
+    closer than `hard_core` to an accepted atom (minimum image).  Candidates
+    are proposed in batches; inside a batch a candidate also yields to any
+    lower-numbered candidate within the hard core.  The species are assigned
+    by a seeded permutation in the given fractions and stored species-major.
+    Returns (float64 torch tensor [n, 3], species counts)."""
+    import torch
+
+    dev = torch.device(device)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    n = int(round(density * side ** 3))
+    cell = hard_core / np.sqrt(3.0)          # at most one atom per grid cell
+    G = max(1, int(np.floor(side / cell)))
+    h = side / G
+    reach = int(np.ceil(hard_core / h))
+    grid = torch.full((G, G, G), -1, dtype=torch.int64, device=dev)
+    pos = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+    r = torch.arange(-reach, reach + 1, device=dev)
+    off = torch.stack(torch.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)   # [K, 3]
+    count = 0
+    for _ in range(max_rounds):
+        if count >= n:
+            break
+        K = min(max(8 * (n - count), 1 << 16), 1 << 17)
+        cand = (torch.rand((K, 3), generator=g, dtype=torch.float64, device=dev) - 0.5) * side
+        ci = torch.clamp(((cand + side / 2) / h).long(), 0, G - 1)
+        nb = (ci[:, None, :] + off[None]) % G                                # [K, 27.., 3]
+        idx = grid[nb[..., 0], nb[..., 1], nb[..., 2]]                          # accepted atoms nearby
+        d = cand[:, None, :] - pos[idx.clamp(min=0)]
+        d -= side * torch.round(d / side)
+        close = ((d * d).sum(-1) < hard_core ** 2) & (idx >= 0)
+        ok = ~close.any(1)
+        cand, ci = cand[ok], ci[ok]
+        if cand.shape[0] == 0:
+            continue
+        # conflicts among this batch's survivors: lowest index wins per cell,
+        # then each survivor yields to a lower-numbered one within the core
+        k = torch.arange(cand.shape[0], device=dev)
+        tmp = torch.full((G * G * G,), cand.shape[0], dtype=torch.int64, device=dev)
+        lin = (ci[:, 0] * G + ci[:, 1]) * G + ci[:, 2]
+        tmp.scatter_reduce_(0, lin, k, reduce="amin")
+        win = tmp[lin] == k
+        cand, ci, k = cand[win], ci[win], k[win]
+        tmp2 = torch.full((G, G, G), -1, dtype=torch.int64, device=dev)
+        tmp2[ci[:, 0], ci[:, 1], ci[:, 2]] = torch.arange(cand.shape[0], device=dev)
+        nb = (ci[:, None, :] + off[None]) % G
+        j = tmp2[nb[..., 0], nb[..., 1], nb[..., 2]]
+        d = cand[:, None, :] - cand[j.clamp(min=0)]
+        d -= side * torch.round(d / side)
+        me = torch.arange(cand.shape[0], device=dev)[:, None]
+        clash = ((d * d).sum(-1) < hard_core ** 2) & (j >= 0) & (j < me)
+        keep = ~clash.any(1)
+        cand, ci = cand[keep], ci[keep]
+        take = min(cand.shape[0], n - count)
+        cand, ci = cand[:take], ci[:take]
+        pos[count:count + take] = cand
+        grid[ci[:, 0], ci[:, 1], ci[:, 2]] = torch.arange(count, count + take, device=dev)
+        count += take
+    pos = pos[:count]
+    n0 = int(round(fractions[0] * count))
+    perm = torch.randperm(count, generator=g, device=dev)
+    counts = np.array([n0, count - n0], np.int64)
+    return pos[perm], counts
+
+
+def amorphous_trajectory(side: float, T: int, sigma: float = 0.05, seed: int = 0, device="cpu", **kw):
+    """T frames of an amorphous box: the RSA configuration plus independent
+    N(0, sigma) Angstrom displacements per frame.  Float32 [T, n, 4] (Angstrom)
+    and the species counts."""
+    import torch
+
+    base, counts = amorphous_box(side, seed=seed, device=device, **kw)
+    g = torch.Generator(device=base.device)
+    g.manual_seed(int(seed) + 7)
+    out = torch.zeros((T, base.shape[0], 4), dtype=torch.float32, device=base.device)
+    for t in range(T):
+        out[t, :, :3] = (base + sigma * torch.randn(base.shape, generator=g, dtype=torch.float64,
+                                                    device=base.device)).to(torch.float32)
+    return out, counts
+
+
+BATCH_MODES = ((32, 32, 32), (64, 64, 32), (64, 64, 64), (128, 128, 32), (128, 128, 64), (256, 256, 32))
+
+
+def batch_jobs(n_jobs: int = 64, seed: int = 20261206, device="cpu", eps: float = 1e-6,
+               perovskite_sides=(8, 48), amorphous_sides=(30.0, 120.0), frames=(1, 64), dt_ps: float = 0.01,
+               modes=BATCH_MODES):
+    """The batched-jobs workload: half perovskite supercells (side drawn in
+    perovskite_sides, phonon trajectories as in the S(Q,w) benchmark, neutron
+    b of Ba/Zr/O), half amorphous binary boxes (side in amorphous_sides,
+    density 0.06 / A^3, 92/8 % species with b of Al/U, 2.2 A hard core);
+    frames and mode boxes drawn per job.  Returns a list of
+    (engine kwargs, trajectory, reduction kwargs)."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(n_jobs):
+        T = int(rng.integers(frames[0], frames[1] + 1))
+        N = tuple(int(v) for v in modes[int(rng.integers(0, len(modes)))])
+        if k % 2 == 0:
+            side = int(rng.integers(perovskite_sides[0], perovskite_sides[1] + 1))
+            pos, counts = phonon_trajectory(side, T, dt_ps=dt_ps, n_phonons=64, rms=0.06, noise=0.01,
+                                            seed=seed + k, device=device)
+            kw = dict(dim=3, n_modes=N, species_counts=counts, eps=eps, precision="fp32",
+                      weights=0.1 * B_NEUTRON_FM, tpe=(2 * np.pi / side,) * 3)
+        else:
+            side = float(rng.uniform(amorphous_sides[0], amorphous_sides[1]))
+            pos, counts = amorphous_trajectory(side, T, seed=seed + k, device=device)
+            kw = dict(dim=3, n_modes=N, species_counts=counts, eps=eps, precision="fp32",
+                      weights=0.1 * B_AL_U_FM, tpe=(2 * np.pi / side,) * 3)
+        nat = float(np.sum(counts))
+        red = dict(n_int=T, win_kind=0, dt=dt_ps, nat=nat, m_lo=0, n_f=max(1, T // 2))
+        jobs.append((kw, pos, red))
+    return jobs
