@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=4)
     ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--workload", choices=("sqw", "batch", "serve"), default="sqw",
+                    help="sqw: the S(Q,w) trajectory benchmark (default line); batch: 64 uneven jobs "
+                         "in one launch per phase; serve: the request loop")
+    ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (5 launches) and exit: "
                          "the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters")
@@ -70,6 +74,12 @@ def main():
 
     if a.copy_mib:
         copy_calibration(a.copy_mib)
+        return
+    if a.workload == "batch":
+        batch_bench(a)
+        return
+    if a.workload == "serve":
+        serve_bench(a)
         return
     from mp_tools_amd import synth
     from mp_tools_amd.sqw import SqwEngine
@@ -250,6 +260,148 @@ def direct_dft_check(a, pos, counts, w, qk, S_sel, geo, n_f, dt, nat):
         out["rel_l2_S_" + k] = float(np.linalg.norm(S - Sd) / np.linalg.norm(Sd))
     out["wall_s"] = time.perf_counter() - t
     return out
+
+
+def batch_bench(a):
+    """64 uneven jobs (half perovskite supercells of side 8-48 cells with
+    phonon trajectories, half amorphous Al/U-like boxes of side 30-120 A with a
+    2.2 A hard core; 1-64 frames; 32^3 .. 256x256x32 modes; FP32, eps 1e-6):
+    spread tiles autotuned per geometry, then all jobs in one launch per phase
+    (S(Q,w) reductions included), captured in a hipGraph and replayed."""
+    import torch
+
+    from mp_tools_amd import synth
+    from mp_tools_amd.sqw import SqwBatch, SqwEngine, autotune_tiles
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    jobs = synth.batch_jobs(n_jobs=a.jobs, seed=a.seed + 2, device=dev, eps=1e-6)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    # autotune every distinct geometry once (cached for the batch's engines)
+    t0 = time.time()
+    tuned = {}
+    for kw, pos, red in jobs:
+        key = tuple(kw["n_modes"])
+        if key in tuned:
+            continue
+        k2 = {k: v for k, v in kw.items() if k not in ("dim", "n_modes", "species_counts")}
+        tuned[key] = autotune_tiles(pos, kw["dim"], kw["n_modes"], kw["species_counts"], **k2)
+    tune_s = time.time() - t0
+    b = SqwBatch([kw for kw, _, _ in jobs])
+    pos = [p for _, p, _ in jobs]
+    reds = [r for _, _, r in jobs]
+    frames = sum(p.shape[0] for p in pos)
+    pts = sum(p.shape[0] * p.shape[1] for p in pos)
+    b.set_timing(True)
+    out = b.run(pos, reds, use_graph=True)      # eager (allocates), stage timings
+    torch.cuda.synchronize()
+    stages = {k: {"ms": v[0], "launches": v[1]} for k, v in b.timing().items() if v[1]}
+    b.run(pos, reds, out=out, use_graph=True)   # capture + first replay
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        b.run(pos, reds, out=out, use_graph=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(a.steps):
+        b.run(pos, reds, out=out, use_graph=True)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t) / a.steps
+    # the same jobs one engine after another (what a caller without batching
+    # does): per job a warm-up call, then the timed call
+    sec_seq, same = 0.0, True
+    for j, (kw, p_, r_) in enumerate(jobs):
+        kw = dict(kw)
+        e = SqwEngine(kw.pop("dim"), kw.pop("n_modes"), kw.pop("species_counts"), **kw)
+        a1 = torch.empty_like(out[0][j])
+        e.amplitudes(p_, out=a1)
+        e.reduce(a1, **r_)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        e.amplitudes(p_, out=a1)
+        e.reduce(a1, **r_)
+        torch.cuda.synchronize()
+        sec_seq += time.perf_counter() - t
+        same = same and bool(torch.equal(a1, out[0][j]))
+        del e, a1
+    torch.cuda.empty_cache()
+    res = {
+        "metric": "batched S(Q,w) jobs: frames/s over all jobs (FP32, eps 1e-6, one launch per phase, hipGraph)",
+        "value": frames / sec, "unit": "frames/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded perovskite phonon trajectories and RSA amorphous boxes)",
+        "config": {"workload": f"{a.jobs} uneven jobs: perovskite side 8-48 cells / amorphous side 30-120 A, "
+                               f"1-64 frames, modes 32^3..256x256x32", "jobs": a.jobs, "frames": frames,
+                   "points": pts, "seed": a.seed + 2},
+        "jobs_per_s": a.jobs / sec, "nu_pts_per_s": pts / sec,
+        "one_by_one_ms": sec_seq * 1e3, "batch_speedup_vs_one_by_one": sec_seq / sec,
+        "batch_equals_one_by_one": bool(same),
+        "stages_eager": stages, "generation_s": gen_s, "autotune_s": tune_s,
+        "tuned_tiles": {"x".join(map(str, k)): [v[0], v[1], v[2]] for k, v in tuned.items()},
+    }
+    print(json.dumps(res))
+
+
+def serve_bench(a):
+    """Serving loop: 4 device-resident trajectories (the S(Q,w) benchmark's
+    40^3 perovskite, 512 frames; a static 128^3 perovskite frame; a 40^3
+    perovskite job of 64 frames and a 120 A amorphous box of 64 frames from
+    the batched workload); Poisson arrivals of map requests (species set,
+    Q box 64^2x8 .. 256^2x32, 1-256 frames, neutron or seeded X-ray form
+    factors) at each rate of the sweep; HBM-budget admission and continuous
+    batching in the native server; throughput and p50/p99 latency."""
+    import torch
+
+    from mp_tools_amd import synth
+    from mp_tools_amd.serve import MapServer, poisson_load
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    p2, c2 = synth.phonon_trajectory(40, 512, dt_ps=0.01, n_phonons=64, rms=0.06, noise=0.01, seed=a.seed,
+                                     device=dev)
+    p3, c3 = synth.static_frame(128, sigma=0.05, seed=a.seed + 3)
+    p3 = torch.tensor(p3, device=dev)
+    p4, c4 = synth.phonon_trajectory(40, 64, dt_ps=0.01, n_phonons=64, rms=0.06, noise=0.01, seed=a.seed + 4,
+                                     device=dev)
+    p5, c5 = synth.amorphous_trajectory(120.0, 64, seed=a.seed + 5, device=dev)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+    A = synth.A_LATTICE
+    trajs = [dict(pos=p2, counts=c2, tpe=(2 * np.pi / 40,) * 3, qs=(2 * np.pi / (40 * A),) * 3,
+                  b=0.1 * synth.B_NEUTRON_FM, nsp=3, T=512),
+             dict(pos=p3, counts=c3, tpe=(2 * np.pi / 128,) * 3, qs=(2 * np.pi / (128 * A),) * 3,
+                  b=0.1 * synth.B_NEUTRON_FM, nsp=3, T=1),
+             dict(pos=p4, counts=c4, tpe=(2 * np.pi / 40,) * 3, qs=(2 * np.pi / (40 * A),) * 3,
+                  b=0.1 * synth.B_NEUTRON_FM, nsp=3, T=64),
+             dict(pos=p5, counts=c5, tpe=(2 * np.pi / 120.0,) * 3, qs=(2 * np.pi / 120.0,) * 3,
+                  b=0.1 * synth.B_AL_U_FM, nsp=2, T=64)]
+    srv = MapServer(hbm_budget=64 << 30, step_budget=8 << 30, max_active=64)
+    for t in trajs:
+        srv.add_trajectory(t["pos"], t["counts"], t["tpe"], t["qs"])
+    # warm-up: one request per trajectory (engines, buffers)
+    poisson_load(srv, trajs, rate=50.0, duration_s=0.1, seed=a.seed)
+    sweep = []
+    for rate in (5.0, 20.0, 50.0, 100.0, 200.0):
+        nreq_target = 40
+        lat, qw, wall, nreq = poisson_load(srv, trajs, rate=rate, duration_s=nreq_target / rate,
+                                           seed=a.seed + int(rate))
+        sweep.append({"rate_req_s": rate, "requests": int(nreq), "throughput_req_s": len(lat) / wall,
+                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "mean_queue_ms": float(np.mean(qw)), "wall_s": wall})
+        print(json.dumps(sweep[-1]), file=sys.stderr, flush=True)
+    best = max(sweep, key=lambda d: d["throughput_req_s"])
+    res = {
+        "metric": "served S(Q,w) maps: throughput (req/s) and p50/p99 latency under Poisson load",
+        "value": best["throughput_req_s"], "unit": "req/s", "n_gpus": 1, "steps": len(sweep), "warmup": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded trajectories; seeded request stream)",
+        "config": {"workload": "4 resident trajectories (40^3x512, 128^3x1, 40^3x64 perovskite; 120 A x64 "
+                               "amorphous); requests: species set, Q box 64^2x8..256^2x32, 1-256 frames, "
+                               "neutron or X-ray weights, eps 1e-6", "requests_per_rate": 40},
+        "sweep": sweep, "server": srv.stats(), "generation_s": gen_s,
+    }
+    srv.close()
+    print(json.dumps(res))
 
 
 def copy_calibration(mib):

@@ -228,6 +228,57 @@ int mpsq_batch_destroy(mpsq_batch b);
 int mpsq_autotune_tiles(const mpsq_sqw_spec *spec, const float *d_pos, int64_t T, int *tile_by, int *tile_bz,
                         double *ms_per_frame);
 
+/* ---------------- S(Q,w) map server ----------------
+ * No counterpart in the reference (an interactive, one-map-at-a-time tool).
+ * Trajectories registered with the server stay resident on the device; map
+ * requests (species set, mode box, frame range, neutron or X-ray weights)
+ * are admitted in arrival order while the admitted requests' amplitudes,
+ * maps and weight tables fit the HBM budget, and a worker thread runs
+ * continuous batching: every step processes the next frames of every
+ * admitted request in one launch per phase, a request joins at the next
+ * step after admission and leaves once its S(Q,w) reduction has run. */
+typedef struct mpsq_server_s *mpsq_server;
+typedef struct mpsq_server_opts {
+    int device;
+    int64_t hbm_budget;    /* bytes for admitted requests' device data (0 = 64 GiB) */
+    int64_t step_budget;   /* bytes of intermediates per step (0 = 8 GiB) */
+    int max_active;        /* requests admitted at once (0 = 64) */
+} mpsq_server_opts;
+int mpsq_server_create(const mpsq_server_opts *o, mpsq_server *srv);
+/* d_pos: device float[T][n_tot][4], owned by the caller while the server
+ * lives; tpe: staging phase scale per axis (2 pi / box length in position
+ * units); qscale: inverse Angstrom per mode index along each axis (2 pi /
+ * box length in Angstrom), used by X-ray form factors. */
+int mpsq_server_add_trajectory(mpsq_server srv, const float *d_pos, int64_t T, int nsp, const int64_t *sp_count,
+                               const double *tpe, const double *qscale, int *traj_id);
+typedef struct mpsq_request {
+    int traj;
+    const int32_t *mask;      /* [nsp] 0/1 species set, or NULL = all */
+    int64_t n_modes[3];
+    int64_t f0, T;            /* frames [f0, f0 + T); one Hann window over the T frames */
+    double eps;
+    int weight_kind;          /* 0: scalar weights w_scalar (neutron 0.1 b); 1: X-ray form factors xff */
+    const double *w_scalar;   /* [nsp] */
+    const double *xff;        /* [nsp][9]: a1..a4, b1..b4, c of c + sum a_k exp(-b_k |Q|^2 / 16 pi^2) */
+    double dt;                /* frame step (ps) */
+    int64_t n_f;              /* frequencies m = 0 .. n_f-1 of S(Q,w) */
+} mpsq_request;
+/* Copies the request; never blocks on the device. */
+int mpsq_server_submit(mpsq_server srv, const mpsq_request *r, int64_t *ticket);
+/* state: 0 queued, 1 admitted, 2 done, or a negative error code; queue_ms:
+ * submit to admission; latency_ms: submit to completion (done only). */
+int mpsq_server_poll(mpsq_server srv, int64_t ticket, int *state, double *queue_ms, double *latency_ms);
+/* Device S [n_f][NQ] and S_tot [NQ] of a done request, valid until release. */
+int mpsq_server_result(mpsq_server srv, int64_t ticket, const double **d_S, const double **d_Stot, int64_t *nq);
+/* Copy a done request's S [n_f][NQ] and S_tot [NQ] into caller device buffers (either may be NULL). */
+int mpsq_server_copy_result(mpsq_server srv, int64_t ticket, double *d_S, double *d_Stot);
+int mpsq_server_release(mpsq_server srv, int64_t ticket);
+/* Block until every submitted request is done or failed. */
+int mpsq_server_wait_idle(mpsq_server srv);
+/* steps run, requests done, summed step launch time on the host (ms) */
+int mpsq_server_stats(mpsq_server srv, int64_t *steps, int64_t *done, double *busy_ms);
+int mpsq_server_destroy(mpsq_server srv);
+
 /* Library version string. */
 const char *mpsq_version(void);
 

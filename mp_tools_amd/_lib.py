@@ -38,6 +38,9 @@ EXPORTS = [
     "mpsq_sqw_set_timing", "mpsq_sqw_get_timing",
     "mpsq_batch_create", "mpsq_batch_run", "mpsq_batch_set_budget", "mpsq_batch_set_timing",
     "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
+    "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
+    "mpsq_server_result", "mpsq_server_copy_result", "mpsq_server_release", "mpsq_server_wait_idle",
+    "mpsq_server_stats", "mpsq_server_destroy",
 ]
 STAGES = ("stage", "sort", "spread_xfft", "colfft", "final_colfft", "reduce")
 
@@ -67,6 +70,17 @@ class SqwSpec(C.Structure):
         ("margin", C.c_float), ("at_ind", C.c_void_p), ("batch_frames", C.c_int),
         ("opts", Opts),
     ]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_int64), ("step_budget", C.c_int64),
+                ("max_active", C.c_int)]
+
+
+class Request(C.Structure):
+    _fields_ = [("traj", C.c_int), ("mask", C.c_void_p), ("n_modes", C.c_int64 * 3), ("f0", C.c_int64),
+                ("T", C.c_int64), ("eps", C.c_double), ("weight_kind", C.c_int), ("w_scalar", C.c_void_p),
+                ("xff", C.c_void_p), ("dt", C.c_double), ("n_f", C.c_int64)]
 
 
 class ReduceSpec(C.Structure):
@@ -137,6 +151,16 @@ def load():
     L.mpsq_batch_get_timing.argtypes = [vp, vp, vp, i32]
     L.mpsq_batch_destroy.argtypes = [vp]
     L.mpsq_autotune_tiles.argtypes = [vp, vp, i64, vp, vp, vp]
+    L.mpsq_server_create.argtypes = [vp, vp]
+    L.mpsq_server_add_trajectory.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp]
+    L.mpsq_server_submit.argtypes = [vp, vp, vp]
+    L.mpsq_server_poll.argtypes = [vp, i64, vp, vp, vp]
+    L.mpsq_server_result.argtypes = [vp, i64, vp, vp, vp]
+    L.mpsq_server_copy_result.argtypes = [vp, i64, vp, vp]
+    L.mpsq_server_release.argtypes = [vp, i64]
+    L.mpsq_server_wait_idle.argtypes = [vp]
+    L.mpsq_server_stats.argtypes = [vp, vp, vp, vp]
+    L.mpsq_server_destroy.argtypes = [vp]
     _lib = L
     return L
 

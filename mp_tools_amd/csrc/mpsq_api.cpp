@@ -7,6 +7,10 @@
 // CPU fallback -- without a usable HIP device every call returns
 // MPSQ_ERR_DEVICE with zeroed outputs.
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -885,6 +889,8 @@ template <typename R> struct Part {
     int b;
     const float *pos;  // frame f0 of the trajectory [T][n_tot][4]
     void *amp;         // output of frame f0 ([b][NQ] combined, [b][nact][NQ] per species)
+    const R *wsc = nullptr;      // species weights for this part (nullptr: the engine's)
+    const float *wtab = nullptr; // per-Q weight table [nact][NQ] (nullptr: the engine's, if any)
 };
 
 template <typename R>
@@ -1178,11 +1184,18 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     ws.G2.ensure(sizeof(Cx<R>) * (size_t)std::max<int64_t>(g2o[np], 1));
     ws.bad.ensure(sizeof(int));
     ws.wc.ensure(sizeof(int) * 8);
-    // ---- spread and column job tables (parts grouped by kernel width)
-    std::vector<int> widths;
+    // ---- spread and column job tables.  Parts are grouped by kernel width
+    // and by how many workgroups of their LDS size fit a CU (a launch's LDS
+    // is the largest of its jobs, so a small-grid job launched with a large
+    // one would run at the large one's occupancy); one launch per group.
+    auto sclass = [&](const Core<R> &c) {
+        const int res = (int)std::min<size_t>(8, (size_t)(156 * 1024) / std::max<size_t>(c.spread_lds_size(), 1));
+        return c.g.w * 16 + res;
+    };
+    std::vector<int> widths;  // group keys
     for (int q = 0; q < np; ++q)
-        if (std::find(widths.begin(), widths.end(), parts[q].e->core.g.w) == widths.end())
-            widths.push_back(parts[q].e->core.g.w);
+        if (std::find(widths.begin(), widths.end(), sclass(parts[q].e->core)) == widths.end())
+            widths.push_back(sclass(parts[q].e->core));
     struct SpreadLaunch {
         int w;
         size_t o_jobs, o_t0, o_xr, lds;
@@ -1199,7 +1212,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         for (int q = 0; q < np; ++q) {
             const Engine<R> &e = *parts[q].e;
             const Core<R> &c = e.core;
-            if (c.g.w != widths[wi]) continue;
+            if (sclass(c) != widths[wi]) continue;
             const int ntr = parts[q].b * e.nact;
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), ws.cs.template as<Cx<R>>(), 0,
                                          ws.binstart.template as<int64_t>() + bso[q], 1,
@@ -1213,7 +1226,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             lds = std::max(lds, c.spread_lds_size());
         }
         sxr[wi] = xcd_ranges(cost, ntl);
-        sl.push_back(SpreadLaunch{widths[wi], 0, 0, 0, lds, (int)sjobs[wi].size()});
+        sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size()});
     }
     std::vector<ColJob<R>> cyj, czj;
     std::vector<int> cyb, czb;
@@ -1228,8 +1241,10 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         Cx<R> *g1 = ws.G1.template as<Cx<R>>() + g1o[q];
         Cx<R> *g2 = ws.G2.template as<Cx<R>>() + g2o[q];
         Cx<R> *out = static_cast<Cx<R> *>(parts[q].amp);
-        const R *wsc = e.combine ? e.wsc.template as<R>() : nullptr;
-        const float *wtab = (e.combine && e.wtab.p) ? e.wtab.template as<float>() : nullptr;
+        const R *wsc = e.combine ? (parts[q].wsc ? parts[q].wsc : e.wsc.template as<R>()) : nullptr;
+        const float *wtab = !e.combine ? nullptr
+                            : parts[q].wtab ? parts[q].wtab
+                            : (e.wtab.p ? e.wtab.template as<float>() : nullptr);
         if (c.dim == 3) {
             ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
             y.blk0 = cyn;
@@ -1509,6 +1524,284 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     if (ms) *ms = best / (double)Tt;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Map server: resident trajectories, HBM-budget admission, continuous batching
+// ---------------------------------------------------------------------------
+struct SrvTraj {
+    const float *pos = nullptr;
+    int64_t T = 0, n_tot = 0;
+    int nsp = 0;
+    std::vector<int64_t> counts;
+    double tpe[3] = {1, 1, 1}, qs[3] = {1, 1, 1};
+};
+
+struct SrvReq {
+    int64_t ticket = 0;
+    mpsq_request r{};
+    std::vector<int32_t> mask;
+    std::vector<double> w, xff;
+    int state = 0;  // 0 queued, 1 admitted, 2 done, < 0 error
+    std::chrono::steady_clock::time_point t_sub, t_admit, t_done;
+    Engine<float> *e = nullptr;
+    int64_t nq = 0;
+    size_t bytes = 0;
+    void *amp = nullptr;
+    double *S = nullptr, *St = nullptr, *coef = nullptr;
+    float *wsc = nullptr, *wtab = nullptr;
+    int64_t next = 0;  // frames already given to a step
+    bool reducing = false;
+    hipEvent_t done_ev = nullptr;
+    mpsq_reduce_spec red{};
+};
+
+struct Server {
+    int device = 0;
+    size_t budget = (size_t)64 << 30, used = 0, step_budget = (size_t)8 << 30;
+    int max_active = 64;
+    hipStream_t st = nullptr;
+    std::vector<SrvTraj> trajs;
+    std::mutex mu;
+    std::condition_variable cv, idle_cv;
+    std::deque<SrvReq *> queue;
+    std::map<int64_t, SrvReq *> reqs;
+    std::vector<SrvReq *> active;  // admitted and not finished (worker thread only)
+    int64_t next_ticket = 1, pending = 0, steps = 0, ndone = 0;
+    double busy_ms = 0.0;
+    bool stop = false;
+    std::thread worker;
+    Workspace<float> ws;
+    Arena ring[3];
+    int ring_k = 0;
+    std::map<std::vector<int64_t>, Engine<float> *> engines;  // by (traj, modes, eps, mask)
+
+    void start(const mpsq_server_opts *o)
+    {
+        device = o ? o->device : 0;
+        if (o && o->hbm_budget > 0) budget = (size_t)o->hbm_budget;
+        if (o && o->step_budget > 0) step_budget = (size_t)o->step_budget;
+        if (o && o->max_active > 0) max_active = o->max_active;
+        HIPCHK(hipSetDevice(device));
+        // non-blocking: the synchronous copies of request set-up (legacy
+        // stream) must not wait for the steps in flight
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        worker = std::thread([this] { loop(); });
+    }
+
+    ~Server()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        if (worker.joinable()) worker.join();
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto &kv : reqs) free_req(kv.second);
+        for (auto &kv : engines) delete kv.second;
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void free_req(SrvReq *q)
+    {
+        void *ptrs[6] = {q->amp, q->S, q->St, q->coef, q->wsc, q->wtab};
+        for (void *p : ptrs)
+            if (p) (void)hipFreeAsync(p, st);
+        if (q->done_ev) (void)hipEventDestroy(q->done_ev);
+        delete q;
+    }
+
+    static size_t req_bytes(const SrvReq &q, int nact, int64_t nq)
+    {
+        size_t b = sizeof(float2) * (size_t)q.r.T * (size_t)nq;               // amplitudes
+        b += sizeof(double) * (size_t)(std::max<int64_t>(q.r.n_f, 1) + 1) * (size_t)nq;  // S, S_tot
+        if (q.r.weight_kind == 1) b += sizeof(float) * (size_t)nact * (size_t)nq;
+        return b + 4096;
+    }
+
+    Engine<float> *engine_for(const SrvReq &q)
+    {
+        const SrvTraj &tj = trajs[q.r.traj];
+        std::vector<int64_t> key{q.r.traj, q.r.n_modes[0], q.r.n_modes[1], q.r.n_modes[2]};
+        int64_t eb;
+        std::memcpy(&eb, &q.r.eps, sizeof(eb));
+        key.push_back(eb);
+        for (int j = 0; j < tj.nsp; ++j) key.push_back(q.mask.empty() ? 1 : q.mask[j]);
+        auto it = engines.find(key);
+        if (it != engines.end()) return it->second;
+        mpsq_sqw_spec sp;
+        mpsq_sqw_default_spec(&sp);
+        sp.dim = 3;
+        for (int d = 0; d < 3; ++d) sp.n_modes[d] = q.r.n_modes[d];
+        sp.eps = q.r.eps;
+        sp.precision = MPSQ_FP32;
+        sp.n_tot = tj.n_tot;
+        sp.nsp = tj.nsp;
+        sp.sp_count = tj.counts.data();
+        sp.mask = q.mask.empty() ? nullptr : q.mask.data();
+        for (int d = 0; d < 3; ++d) sp.tpe[d] = tj.tpe[d];
+        sp.opts.device = device;
+        Engine<float> *e = new Engine<float>();
+        try {
+            e->create(&sp);
+        } catch (...) {
+            delete e;
+            throw;
+        }
+        engines[key] = e;
+        return e;
+    }
+
+    // set-up of an admitted request (worker thread)
+    void prepare(SrvReq *q)
+    {
+        Engine<float> *e = engine_for(*q);
+        q->e = e;
+        q->nq = e->nq();
+        const int nact = e->nact;
+        HIPCHK(hipMallocAsync(&q->amp, sizeof(float2) * (size_t)q->r.T * (size_t)q->nq, st));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->S),
+                              sizeof(double) * (size_t)std::max<int64_t>(q->r.n_f, 1) * (size_t)q->nq, st));
+        HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->St), sizeof(double) * (size_t)q->nq, st));
+        // weights of the active species, in the engine's active order
+        std::vector<float> wa;
+        std::vector<double> ca;
+        const SrvTraj &tj = trajs[q->r.traj];
+        for (int j = 0; j < tj.nsp; ++j) {
+            if (!q->mask.empty() && !q->mask[j]) continue;
+            wa.push_back(q->r.weight_kind == 0 ? (float)q->w[j] : 1.0f);
+            if (q->r.weight_kind == 1) ca.insert(ca.end(), q->xff.begin() + 9 * j, q->xff.begin() + 9 * j + 9);
+        }
+        HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->wsc), sizeof(float) * wa.size(), st));
+        HIPCHK(hipMemcpyAsync(q->wsc, wa.data(), sizeof(float) * wa.size(), hipMemcpyHostToDevice, st));
+        if (q->r.weight_kind == 1) {
+            HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->coef), sizeof(double) * ca.size(), st));
+            HIPCHK(hipMemcpyAsync(q->coef, ca.data(), sizeof(double) * ca.size(), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->wtab), sizeof(float) * (size_t)nact * q->nq, st));
+            launch_xray_table(q->r.n_modes, tj.qs, nact, q->coef, q->wtab, st);
+        }
+        HIPCHK(hipEventCreateWithFlags(&q->done_ev, hipEventDisableTiming));
+        double nat = 0.0;
+        for (int64_t c : tj.counts) nat += (double)c;
+        q->red.n_int = q->r.T;
+        q->red.win_kind = 0;
+        q->red.dt = q->r.dt;
+        q->red.nat = nat;
+        q->red.m_lo = 0;
+        q->red.n_f = q->r.n_f;
+        q->red.subtract_mean = 1;
+        q->red.d_qfac = nullptr;
+    }
+
+    void loop()
+    {
+        (void)hipSetDevice(device);
+        for (;;) {
+            std::vector<SrvReq *> admit;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                if (stop) return;
+                while (!queue.empty() && (int)active.size() + (int)admit.size() < max_active) {
+                    SrvReq *q = queue.front();
+                    if (used + q->bytes > budget) break;  // arrival order: the head waits for memory
+                    queue.pop_front();
+                    used += q->bytes;
+                    admit.push_back(q);
+                }
+                if (admit.empty() && active.empty()) {
+                    cv.wait_for(lk, std::chrono::milliseconds(2));
+                    continue;
+                }
+            }
+            for (SrvReq *q : admit) {
+                int err = 0;
+                try {
+                    prepare(q);
+                } catch (const Err &er) {
+                    err = er.code;
+                } catch (...) {
+                    err = MPSQ_ERR_DEVICE;
+                }
+                std::lock_guard<std::mutex> lk(mu);
+                q->t_admit = std::chrono::steady_clock::now();
+                if (err) {
+                    q->state = -err;
+                    used -= q->bytes;
+                    --pending;
+                    idle_cv.notify_all();
+                } else {
+                    q->state = 1;
+                    active.push_back(q);
+                }
+            }
+            // one continuous-batching step: the next frames of every admitted
+            // request that still has frames, in one launch per phase
+            const auto t0 = std::chrono::steady_clock::now();
+            std::vector<Part<float>> parts;
+            std::vector<SrvReq *> in_step, finishing;
+            int nrun = 0;
+            for (SrvReq *q : active)
+                if (!q->reducing) ++nrun;
+            for (SrvReq *q : active) {
+                if (q->reducing) continue;
+                const SrvTraj &tj = trajs[q->r.traj];
+                const size_t share = step_budget / (size_t)std::max(nrun, 1);
+                const int64_t fit = (int64_t)(share / std::max<size_t>(q->e->bytes_per_frame(), 1));
+                const int b = (int)std::max<int64_t>(1, std::min<int64_t>(fit, q->r.T - q->next));
+                Part<float> p;
+                p.e = q->e;
+                p.f0 = q->r.f0 + q->next;
+                p.b = b;
+                p.pos = tj.pos + (size_t)p.f0 * tj.n_tot * 4;
+                p.amp = static_cast<float2 *>(q->amp) + (size_t)q->next * q->nq;
+                p.wsc = q->wsc;
+                p.wtab = q->wtab;
+                parts.push_back(p);
+                in_step.push_back(q);
+                q->next += b;
+                if (q->next == q->r.T) finishing.push_back(q);
+            }
+            try {
+                if (!parts.empty()) run_parts<float>(parts, ws, ring[ring_k++ % 3], st, nullptr);
+                for (SrvReq *q : finishing) {
+                    q->e->reduce_on(q->amp, q->r.T, &q->red, q->S, q->St, st, false);
+                    HIPCHK(hipEventRecord(q->done_ev, st));
+                    q->reducing = true;
+                }
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu);
+                for (SrvReq *q : in_step) q->state = -MPSQ_ERR_DEVICE;
+            }
+            if (!parts.empty()) {
+                ++steps;
+                busy_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            }
+            // completions; the next step is already queued behind them
+            bool any_running = false;
+            for (SrvReq *q : active) any_running = any_running || (!q->reducing && q->state > 0);
+            std::vector<SrvReq *> still;
+            for (SrvReq *q : active) {
+                bool fin = q->state < 0;
+                if (!fin && q->reducing) {
+                    if (!any_running) (void)hipEventSynchronize(q->done_ev);
+                    fin = hipEventQuery(q->done_ev) == hipSuccess;
+                }
+                if (fin) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (q->state > 0) q->state = 2;
+                    else used -= q->bytes;
+                    q->t_done = std::chrono::steady_clock::now();
+                    ++ndone;
+                    --pending;
+                    idle_cv.notify_all();
+                } else {
+                    still.push_back(q);
+                }
+            }
+            active.swap(still);
+        }
+    }
+};
 
 }  // namespace mpsq
 
@@ -1931,6 +2224,189 @@ int mpsq_batch_get_timing(mpsq_batch b, double *ms, int64_t *launches, int n)
 int mpsq_batch_destroy(mpsq_batch b)
 {
     delete b;
+    return 0;
+}
+
+// ---- map server ----
+struct mpsq_server_s {
+    Server *s = nullptr;
+    ~mpsq_server_s() { delete s; }
+};
+
+int mpsq_server_create(const mpsq_server_opts *o, mpsq_server *srv)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    *srv = nullptr;
+    mpsq_server_s *h = new (std::nothrow) mpsq_server_s();
+    if (!h) return MPSQ_ERR_DEVICE;
+    try {
+        h->s = new Server();
+        h->s->start(o);
+    } catch (const Err &er) {
+        delete h;
+        return er.code;
+    } catch (...) {
+        delete h;
+        return MPSQ_ERR_DEVICE;
+    }
+    *srv = h;
+    return 0;
+}
+
+int mpsq_server_add_trajectory(mpsq_server srv, const float *d_pos, int64_t T, int nsp, const int64_t *sp_count,
+                               const double *tpe, const double *qscale, int *traj_id)
+{
+    if (!srv || !d_pos || T < 1 || nsp < 1 || nsp > MAXSP || !sp_count || !tpe) return MPSQ_ERR_BAD_ARG;
+    SrvTraj t;
+    t.pos = d_pos;
+    t.T = T;
+    t.nsp = nsp;
+    t.counts.assign(sp_count, sp_count + nsp);
+    for (int64_t c : t.counts) {
+        if (c < 0) return MPSQ_ERR_BAD_ARG;
+        t.n_tot += c;
+    }
+    for (int d = 0; d < 3; ++d) {
+        t.tpe[d] = tpe[d];
+        t.qs[d] = qscale ? qscale[d] : 1.0;
+    }
+    std::lock_guard<std::mutex> lk(srv->s->mu);
+    srv->s->trajs.push_back(t);
+    if (traj_id) *traj_id = (int)srv->s->trajs.size() - 1;
+    return 0;
+}
+
+int mpsq_server_submit(mpsq_server srv, const mpsq_request *r, int64_t *ticket)
+{
+    if (!srv || !r || !ticket) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    if (r->traj < 0 || r->traj >= (int)S.trajs.size()) return MPSQ_ERR_BAD_ARG;
+    const SrvTraj &tj = S.trajs[r->traj];
+    if (r->T < 1 || r->f0 < 0 || r->f0 + r->T > tj.T || r->n_f < 0 || !(r->eps > 0)) return MPSQ_ERR_BAD_ARG;
+    for (int d = 0; d < 3; ++d)
+        if (r->n_modes[d] < 1) return MPSQ_ERR_BAD_ARG;
+    if (r->weight_kind == 0 ? !r->w_scalar : (r->weight_kind != 1 || !r->xff)) return MPSQ_ERR_BAD_ARG;
+    SrvReq *q = new SrvReq();
+    q->r = *r;
+    if (r->mask) q->mask.assign(r->mask, r->mask + tj.nsp);
+    int nact = 0;
+    for (int j = 0; j < tj.nsp; ++j) nact += q->mask.empty() ? 1 : (q->mask[j] != 0);
+    if (nact == 0) {
+        delete q;
+        return MPSQ_ERR_BAD_ARG;
+    }
+    if (r->weight_kind == 0) q->w.assign(r->w_scalar, r->w_scalar + tj.nsp);
+    else q->xff.assign(r->xff, r->xff + 9 * tj.nsp);
+    q->r.mask = nullptr;
+    q->r.w_scalar = nullptr;
+    q->r.xff = nullptr;
+    q->bytes = Server::req_bytes(*q, nact, r->n_modes[0] * r->n_modes[1] * r->n_modes[2]);
+    if (q->bytes > S.budget) {
+        delete q;
+        return MPSQ_ERR_SIZE;
+    }
+    q->ticket = S.next_ticket++;
+    q->t_sub = std::chrono::steady_clock::now();
+    S.reqs[q->ticket] = q;
+    S.queue.push_back(q);
+    ++S.pending;
+    *ticket = q->ticket;
+    S.cv.notify_all();
+    return 0;
+}
+
+int mpsq_server_poll(mpsq_server srv, int64_t ticket, int *state, double *queue_ms, double *latency_ms)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    auto it = S.reqs.find(ticket);
+    if (it == S.reqs.end()) return MPSQ_ERR_BAD_ARG;
+    const SrvReq *q = it->second;
+    if (state) *state = q->state;
+    using ms = std::chrono::duration<double, std::milli>;
+    if (queue_ms) *queue_ms = q->state == 0 ? -1.0 : ms(q->t_admit - q->t_sub).count();
+    if (latency_ms) *latency_ms = q->state == 2 ? ms(q->t_done - q->t_sub).count() : -1.0;
+    return 0;
+}
+
+int mpsq_server_result(mpsq_server srv, int64_t ticket, const double **d_S, const double **d_Stot, int64_t *nq)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    auto it = S.reqs.find(ticket);
+    if (it == S.reqs.end() || it->second->state != 2) return MPSQ_ERR_NOT_SET;
+    if (d_S) *d_S = it->second->S;
+    if (d_Stot) *d_Stot = it->second->St;
+    if (nq) *nq = it->second->nq;
+    return 0;
+}
+
+int mpsq_server_copy_result(mpsq_server srv, int64_t ticket, double *d_S, double *d_Stot)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    const double *src_S = nullptr, *src_St = nullptr;
+    int64_t nq = 0, nf = 0;
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        auto it = S.reqs.find(ticket);
+        if (it == S.reqs.end() || it->second->state != 2) return MPSQ_ERR_NOT_SET;
+        src_S = it->second->S;
+        src_St = it->second->St;
+        nq = it->second->nq;
+        nf = it->second->r.n_f;
+    }
+    if (hipSetDevice(S.device) != hipSuccess) return MPSQ_ERR_DEVICE;
+    if (d_S && nf > 0 && hipMemcpy(d_S, src_S, sizeof(double) * (size_t)(nf * nq), hipMemcpyDeviceToDevice))
+        return MPSQ_ERR_DEVICE;
+    if (d_Stot && hipMemcpy(d_Stot, src_St, sizeof(double) * (size_t)nq, hipMemcpyDeviceToDevice))
+        return MPSQ_ERR_DEVICE;
+    return 0;
+}
+
+int mpsq_server_release(mpsq_server srv, int64_t ticket)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    auto it = S.reqs.find(ticket);
+    if (it == S.reqs.end()) return MPSQ_ERR_BAD_ARG;
+    SrvReq *q = it->second;
+    if (q->state == 0 || q->state == 1) return MPSQ_ERR_NOT_SET;  // queued or running
+    if (q->state == 2) S.used -= q->bytes;
+    (void)hipSetDevice(S.device);
+    S.reqs.erase(it);
+    S.free_req(q);
+    S.cv.notify_all();
+    return 0;
+}
+
+int mpsq_server_wait_idle(mpsq_server srv)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::unique_lock<std::mutex> lk(S.mu);
+    S.idle_cv.wait(lk, [&] { return S.pending == 0; });
+    return 0;
+}
+
+int mpsq_server_stats(mpsq_server srv, int64_t *steps, int64_t *done, double *busy_ms)
+{
+    if (!srv) return MPSQ_ERR_BAD_ARG;
+    Server &S = *srv->s;
+    std::lock_guard<std::mutex> lk(S.mu);
+    if (steps) *steps = S.steps;
+    if (done) *done = S.ndone;
+    if (busy_ms) *busy_ms = S.busy_ms;
+    return 0;
+}
+
+int mpsq_server_destroy(mpsq_server srv)
+{
+    delete srv;
     return 0;
 }
 

@@ -182,6 +182,8 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
                         hipStream_t st);
 
 // --- launch helpers (sqw_kernels.hip) ---
+// X-ray form-factor weights w[s][Q] on the mode grid N (see sqw_kernels.hip)
+void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double *coef, float *w, hipStream_t st);
 template <typename R> int sqw_tile_elems();
 template <typename R>
 void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,

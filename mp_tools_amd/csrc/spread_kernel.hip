@@ -39,6 +39,8 @@
 // round robin) runs a contiguous range of tiles: neighbouring tiles share
 // most of their candidates, which then come from that XCD's L2.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "mpsq_kernels.h"
 
@@ -552,17 +554,22 @@ static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, con
 {
     constexpr int E = SpreadCfg<R>::E, NT = SpreadCfg<R>::NT;
     const void *fn = reinterpret_cast<const void *>(&k_spread_xfft<R, E, NT, W>);
-    static int resident = 0;  // workgroups per CU at this LDS size
-    static size_t resident_lds = 0;
+    static std::mutex mu;
+    static std::map<size_t, int> resident_at;  // workgroups per CU at an LDS size
     static int ncu = 0;
-    if (resident == 0 || resident_lds != lds) {
-        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
-        int dev = 0, nb = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, lds) != hipSuccess || nb < 1) nb = 1;
-        resident = nb;
-        resident_lds = lds;
+    int resident;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = resident_at.find(lds);
+        if (it == resident_at.end()) {
+            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
+            int dev = 0, nb = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, lds) != hipSuccess || nb < 1) nb = 1;
+            it = resident_at.emplace(lds, nb).first;
+        }
+        resident = it->second;
     }
     int grid = std::max(1, ncu) * resident;
     grid = ((grid + 7) / 8) * 8;

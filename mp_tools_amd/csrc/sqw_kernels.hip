@@ -200,6 +200,35 @@ __global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename
     }
 }
 
+// Per-Q X-ray weights of nsp species on the mode grid (CMCL order, k1
+// fastest, k_d = i_d - N_d/2): w[s][Q] = c_s + sum_{k=1..4} a_sk exp(-b_sk |Q|^2 / (16 pi^2)),
+// |Q|^2 = sum_d (k_d qs_d)^2 in inverse Angstrom (qs_d = 2 pi / box length
+// along d); coef[s] = {a1..a4, b1..b4, c}
+__global__ void k_xray_table(int64_t N1, int64_t N2, int64_t N3, double qs1, double qs2, double qs3, int nsp,
+                             const double *__restrict__ coef, float *__restrict__ w)
+{
+    const int64_t NQ = N1 * N2 * N3;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NQ) return;
+    const int64_t i1 = q % N1, i2 = (q / N1) % N2, i3 = q / (N1 * N2);
+    const double k1 = (double)(i1 - N1 / 2) * qs1, k2 = (double)(i2 - N2 / 2) * qs2, k3 = (double)(i3 - N3 / 2) * qs3;
+    const double s2 = (k1 * k1 + k2 * k2 + k3 * k3) / (16.0 * M_PI * M_PI);
+    for (int s = 0; s < nsp; ++s) {
+        const double *c = coef + 9 * s;
+        double f = c[8];
+        for (int k = 0; k < 4; ++k) f += c[k] * exp(-c[4 + k] * s2);
+        w[(int64_t)s * NQ + q] = (float)f;
+    }
+}
+
+void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double *coef, float *w, hipStream_t st)
+{
+    const int64_t NQ = N[0] * N[1] * N[2];
+    if (NQ <= 0 || nsp <= 0) return;
+    hipLaunchKernelGGL(k_xray_table, dim3((unsigned)((NQ + 255) / 256)), dim3(256), 0, st, N[0], N[1], N[2], qs[0],
+                       qs[1], qs[2], nsp, coef, w);
+}
+
 template <typename R> struct SqwTiles;
 template <> struct SqwTiles<float> { static constexpr int E = 32, NT = 256; };
 template <> struct SqwTiles<double> { static constexpr int E = 16, NT = 256; };
