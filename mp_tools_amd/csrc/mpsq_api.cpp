@@ -439,7 +439,7 @@ template <typename R> struct Core {
         // spread tile: whole x extent x (by x bz) rows within the LDS tile;
         // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
         // one tile contiguous
-        const int rows_max = (int)std::min<int64_t>(128, TE / n[0]);
+        const int rows_max = spread_max_rows<R>((int)n[0]);
         const int bmax = spread_max_tile_b();
         int by = 1, bz = 1;
         if (dim == 2) {
@@ -467,7 +467,7 @@ template <typename R> struct Core {
         }
         if (o->tile_by > 0 && dim >= 2 && n[1] % o->tile_by == 0) by = o->tile_by;
         if (o->tile_bz > 0 && dim >= 3 && n[2] % o->tile_bz == 0) bz = o->tile_bz;
-        if ((int64_t)by * bz * n[0] > TE || by > bmax || bz > bmax) throw Err{MPSQ_ERR_BAD_ARG};
+        if (by * bz > rows_max || by > bmax || bz > bmax) throw Err{MPSQ_ERR_BAD_ARG};
         if (spread_lds<R>(by * bz, (int)n[0], w) > 156 * 1024) throw Err{MPSQ_ERR_SIZE};
         if ((dim >= 2 && by + w > n[1]) || (dim >= 3 && bz + w > n[2])) throw Err{MPSQ_ERR_BAD_ARG};
         g.by = by;
@@ -571,13 +571,20 @@ template <typename R> struct Core {
         const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
         return ((dim >= 2) ? g.n2 / by : 1) * ((dim >= 3) ? g.n3 / bz : 1);
     }
+    // x segments of the covering bound: 2^sxsh >= w cells, at most 64 of them
+    void segments(int &sxsh, int &nseg) const
+    {
+        sxsh = 0;
+        while ((1 << sxsh) < g.w || ((g.n1 + (1 << sxsh) - 1) >> sxsh) > 64) ++sxsh;
+        nseg = (g.n1 + (1 << sxsh) - 1) >> sxsh;
+    }
     size_t spread_lds_size() const
     {
         const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
         return spread_lds<R>(by * bz, g.n1, g.w);
     }
     SpreadJob<R> spreadjob(int ntr, const PtRec<R> *srec, const Cx<R> *cs, int64_t c_tstride, const int64_t *bstart,
-                           int bin_tstride, Cx<R> *G1, const float *amax, int final_conj) const
+                           const int *bmax, int bin_tstride, Cx<R> *G1, const float *amax, int final_conj) const
     {
         SpreadJob<R> J{};
         J.g = g;
@@ -591,11 +598,14 @@ template <typename R> struct Core {
         J.phinv1 = phinv[0].template as<R>();
         J.G1 = G1;
         J.amax = amax;
+        J.binmax = bmax;
         J.final_conj = final_conj;
         J.ntr = ntr;
         J.ntiles = ntiles();
         J.tile0 = 0;
-        const int by = (dim >= 2) ? g.by : 1;
+        const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
+        J.grp = spread_group_rows<R>(by * bz, g.n1);
+        J.grp_sh = ilog2_ceil(J.grp);
         J.dntr = make_fdiv((uint32_t)ntr);
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
@@ -626,7 +636,7 @@ template <typename R> struct Plan {
     Core<R> core;
     int ntrans = 1;
     int64_t M = -1;
-    DevBuf xyz[3], rec, key, srec, perm, H, Hs, bsum, binstart, bad, wc;
+    DevBuf xyz[3], rec, key, srec, perm, H, Hs, bsum, binstart, binmax, bad, wc, fbl;
     DevBuf cin, cs, G1, G2, fout, amax;
     Arena sort_tab, run_tab;
     int nch = 0;
@@ -681,6 +691,7 @@ template <typename R> struct Plan {
         tinfo[0].bs0 = 0;
         tinfo[0].nch = nch;
         tinfo[0].nbins = c.g.nbins;
+        c.segments(tinfo[0].sxsh, tinfo[0].nseg);
         sort_tab.begin();
         off_chunks = sort_tab.put(chunks);
         off_tinfo = sort_tab.put(tinfo);
@@ -697,6 +708,11 @@ template <typename R> struct Plan {
                                srec.as<PtRec<R>>(), nullptr, perm.as<int>(), c.st);
         launch_binstart(Hs.as<int64_t>(), sort_tab.dev<TransformInfo>(off_tinfo), 1, c.g.nbins,
                         binstart.as<int64_t>(), c.st);
+        if (sizeof(R) == 4) {
+            binmax.ensure(sizeof(int) * (size_t)(c.g.nbins + 1));
+            launch_binmax<R>(srec.as<PtRec<R>>(), binstart.as<int64_t>(), sort_tab.dev<TransformInfo>(off_tinfo), 1,
+                             c.g.nbins, binmax.as<int>(), c.st);
+        }
         HIPCHK(hipGetLastError());
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
@@ -744,7 +760,8 @@ template <typename R> struct Plan {
         }
         // one-entry job tables: spread, then the column passes
         std::vector<SpreadJob<R>> sj(1, c.spreadjob(ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
-                                                     binstart.as<int64_t>(), 0, g1, am,
+                                                     binstart.as<int64_t>(),
+                                                     sizeof(R) == 4 ? binmax.as<int>() : nullptr, 0, g1, am,
                                                      (c.dim == 1 && c.iflag > 0) ? 1 : 0));
         std::vector<int> zero1(1, 0);
         const int ntile = sj[0].ntiles * ntrans;
@@ -760,9 +777,10 @@ template <typename R> struct Plan {
         const size_t o_sj = run_tab.put(sj), o_z = run_tab.put(zero1), o_xr = run_tab.put(xr);
         const size_t o_cy = run_tab.put(cy), o_cz = run_tab.put(cz);
         run_tab.upload(c.st);
-        wc.ensure(sizeof(int) * 8);
+        wc.ensure(sizeof(int) * XCD_CTR_INTS);
+        fbl.ensure(sizeof(int) * 2 * (size_t)ntile);
         launch_spread_jobs<R>(run_tab.dev<SpreadJob<R>>(o_sj), run_tab.dev<int>(o_z), 1, c.g.w,
-                              run_tab.dev<int>(o_xr), wc.as<int>(), c.spread_lds_size(), c.st);
+                              run_tab.dev<int>(o_xr), wc.as<int>(), fbl.as<int>(), c.spread_lds_size(), c.st);
         if (!cy.empty())
             launch_colfft_jobs<R>(run_tab.dev<ColJob<R>>(o_cy), run_tab.dev<int>(o_z), 1,
                                   (int)Core<R>::col_blocks(cy[0]), false, colfft_lds<R>(cy[0].cp, false), c.st);
@@ -878,7 +896,7 @@ template <typename R> struct Engine;
 // sorted), the sort's count matrix, bin starts, the two intermediate grids,
 // and the job tables.
 template <typename R> struct Workspace {
-    DevBuf rec, key, cstr, srec, cs, H, Hs, bsum, binstart, G1, G2, bad, wc;
+    DevBuf rec, key, cstr, srec, cs, H, Hs, bsum, binstart, binmax, G1, G2, bad, wc, fbl;
     Arena tab;
 };
 
@@ -1164,6 +1182,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                 ti.bs0 = bso[q] + (int64_t)(fb * e.nact + a) * (c.g.nbins + 1);
                 ti.nch = nch;
                 ti.nbins = c.g.nbins;
+                c.segments(ti.sxsh, ti.nseg);
                 tinfo.push_back(ti);
                 hsz += (int64_t)c.g.nbins * nch;
             }
@@ -1180,10 +1199,11 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     ws.Hs.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(hsz, 1));
     ws.bsum.ensure(sizeof(int64_t) * (size_t)scan_bsum_size(hsz));
     ws.binstart.ensure(sizeof(int64_t) * (size_t)std::max<int64_t>(bso[np], 1));
+    if (sizeof(R) == 4) ws.binmax.ensure(sizeof(int) * (size_t)std::max<int64_t>(bso[np], 1));
     ws.G1.ensure(sizeof(Cx<R>) * (size_t)std::max<int64_t>(g1o[np], 1));
     ws.G2.ensure(sizeof(Cx<R>) * (size_t)std::max<int64_t>(g2o[np], 1));
     ws.bad.ensure(sizeof(int));
-    ws.wc.ensure(sizeof(int) * 8);
+    ws.wc.ensure(sizeof(int) * XCD_CTR_INTS);
     // ---- spread and column job tables.  Parts are grouped by kernel width
     // and by how many workgroups of their LDS size fit a CU (a launch's LDS
     // is the largest of its jobs, so a small-grid job launched with a large
@@ -1202,6 +1222,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         int njobs;
     };
     std::vector<SpreadLaunch> sl;
+    int max_tiles = 0;  // tiles of the largest spread launch
     std::vector<std::vector<SpreadJob<R>>> sjobs(widths.size());
     std::vector<std::vector<int>> st0(widths.size()), sxr(widths.size());
     for (size_t wi = 0; wi < widths.size(); ++wi) {
@@ -1215,7 +1236,8 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             if (sclass(c) != widths[wi]) continue;
             const int ntr = parts[q].b * e.nact;
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), ws.cs.template as<Cx<R>>(), 0,
-                                         ws.binstart.template as<int64_t>() + bso[q], 1,
+                                         ws.binstart.template as<int64_t>() + bso[q],
+                                         sizeof(R) == 4 ? ws.binmax.template as<int>() + bso[q] : nullptr, 1,
                                          ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
             J.tile0 = tile;
             tile += J.ntiles * ntr;
@@ -1227,7 +1249,9 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         }
         sxr[wi] = xcd_ranges(cost, ntl);
         sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size()});
+        max_tiles = std::max(max_tiles, tile);
     }
+    ws.fbl.ensure(sizeof(int) * 2 * (size_t)std::max(max_tiles, 1));
     std::vector<ColJob<R>> cyj, czj;
     std::vector<int> cyb, czb;
     int cyn = 0, czn = 0;
@@ -1292,11 +1316,14 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                            ws.cs.template as<Cx<R>>(), nullptr, st);
     launch_binstart(ws.Hs.template as<int64_t>(), A.dev<TransformInfo>(o_ti), (int)ntr_all, max_nbins,
                     ws.binstart.template as<int64_t>(), st);
+    if (sizeof(R) == 4)
+        launch_binmax<R>(ws.srec.template as<PtRec<R>>(), ws.binstart.template as<int64_t>(),
+                         A.dev<TransformInfo>(o_ti), (int)ntr_all, max_nbins, ws.binmax.template as<int>(), st);
     tm->end(ev, st);
     ev = tm->begin(ST_SPREAD, st);
     for (const SpreadLaunch &L : sl)
         launch_spread_jobs<R>(A.dev<SpreadJob<R>>(L.o_jobs), A.dev<int>(L.o_t0), L.njobs, L.w, A.dev<int>(L.o_xr),
-                              ws.wc.template as<int>(), L.lds, st);
+                              ws.wc.template as<int>(), ws.fbl.template as<int>(), L.lds, st);
     tm->end(ev, st);
     if (cyn > 0) {
         ev = tm->begin(ST_COLPASS, st);
@@ -1480,7 +1507,7 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     probe.create(spec);
     const Core<R> &c = probe.core;
     const int dim = c.dim, w = c.g.w, n1 = c.g.n1, n2 = c.g.n2, n3 = c.g.n3;
-    const int TE = spread_tile_elems<R>(), bmax = spread_max_tile_b();
+    const int bmax = spread_max_tile_b();
     DevBuf amp;
     amp.ensure(sizeof(Cx<R>) * (size_t)Tt * (size_t)probe.amp_frame_elems());
     hipEvent_t e0, e1;
@@ -1490,7 +1517,7 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     int bb[2] = {c.g.by, c.g.bz};
     for (int by = 1; by <= (dim >= 2 ? bmax : 1); by *= 2)
         for (int bz = 1; bz <= (dim >= 3 ? bmax : 1); bz *= 2) {
-            if ((int64_t)by * bz * n1 > TE) continue;
+            if (by * bz > spread_max_rows<R>(n1)) continue;
             if (dim >= 2 && (n2 % by || by + w > n2)) continue;
             if (dim >= 3 && (n3 % bz || bz + w > n3)) continue;
             if (spread_lds<R>(by * bz, n1, w) > 156 * 1024) continue;

@@ -61,6 +61,7 @@ struct TransformInfo {
     int64_t p0, p1, hbase;
     int64_t bs0;      // first entry of the transform's binstart block (nbins + 1 entries)
     int nch, nbins;
+    int sxsh, nseg;   // x segments of the covering bound: 2^sxsh cells (>= w), nseg of them
 };
 
 struct ColPass {
@@ -77,6 +78,11 @@ struct ColPass {
 // geometries, frame counts, species); a single-job run is a one-entry table.
 // ---------------------------------------------------------------------------
 constexpr int MAX_JOBS = 256;   // jobs of one launch (tile-id prefix cached in LDS)
+// counters of the persistent spread, one 256-byte line each (a line shared
+// by the 8 XCDs' L2s serialises every atomic on it): 8 per-XCD tile
+// counters, then the count of queued wide passes and their take counter
+constexpr int XCD_CTR_STRIDE = 64;
+constexpr int XCD_CTR_INTS = 10 * XCD_CTR_STRIDE;
 
 // trajectory staging of one job's batch of frames: points [out0, out0+npts)
 // of the shared point buffers, blocks [blk0, blk0 + ceil(npts/256))
@@ -109,6 +115,8 @@ template <typename R> struct SpreadJob {
     const R *phinv1;
     typename cplx<R>::T *G1;             // [ntr][n3][n2][N1]
     const float *amax;                   // per transform, or nullptr (strengths bounded by 1)
+    const int *binmax;                   // FP32: per-bin covering bound, indexed like binstart (or nullptr)
+    int grp, grp_sh;                     // rows per x-FFT group (a power of two) and its log2
     int final_conj;
     int ntr, ntiles, tile0;
     FDiv dntr, dnty, dby;
@@ -162,16 +170,28 @@ void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::
 template <typename R> int spread_tile_elems();
 int spread_max_runs();
 int spread_max_tile_b();
+// largest tile (rows of the whole x extent) of a fine grid of n1 cells in x,
+// and the rows one x-FFT group of such a tile holds
+template <typename R> int spread_max_rows(int n1);
+template <typename R> int spread_group_rows(int rows, int n1);
 template <typename R> size_t spread_lds(int rows, int n1, int w);
+// binmax[bs0 + b] = largest number of bin b's points whose first kernel cell
+// lies in 3 cyclically consecutive x segments (a bound on how many of them
+// cover any one cell of a row)
+template <typename R>
+void launch_binmax(const PtRec<R> *srec, const int64_t *binstart, const TransformInfo *tinfo, int ntr, int max_nbins,
+                   int *binmax, hipStream_t st);
 template <typename R> int colfft_tile_elems(bool final);
 // Spread + x FFT of every tile of a job table in one persistent launch.
 // Workgroups run on XCD blockIdx.x % 8 and take tiles of their XCD's range
-// [xr[x], xr[x+1]) of the tile-id space from the counter wc[x] (zeroed by
-// this call).  All jobs share the kernel width w; lds = max over jobs of
-// spread_lds.
+// [xr[x], xr[x+1]) of the tile-id space from the counter
+// wc[x * XCD_CTR_STRIDE] (wc: XCD_CTR_INTS ints, zeroed by this call).  FP32:
+// tiles too dense for 32-bit accumulation are queued in fbl (2 ints per
+// tile of the table) and run by a second, 64-bit launch.  All jobs share the
+// kernel width w; lds = max over jobs of spread_lds.
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
-                        size_t lds, hipStream_t st);
+                        int *fbl, size_t lds, hipStream_t st);
 // p[0..n) = 0 by a kernel (graph-capturable, unlike a memset node)
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)

@@ -350,6 +350,40 @@ __global__ void k_binstart(const int64_t *__restrict__ Hs, const TransformInfo *
     binstart[ti.bs0 + b] = (b < ti.nbins && ti.nch > 0) ? Hs[ti.hbase + (int64_t)b * ti.nch] : ti.p1;
 }
 
+// Covering bound of the FP32 spread (one wave per bin): a cell x of a row is
+// covered by the points whose first kernel cell lies in [x - w + 1, x]
+// (cyclically), a window of w <= 2^sxsh cells that meets at most 3
+// cyclically consecutive segments of 2^sxsh cells (the last segment may be
+// short).  The largest count over such segment triples bounds the number of
+// the bin's points covering any cell.
+template <typename R>
+__global__ void k_binmax(const PtRec<R> *__restrict__ srec, const int64_t *__restrict__ binstart,
+                         const TransformInfo *__restrict__ tinfo, int *__restrict__ binmax)
+{
+    __shared__ int hist[4][64];
+    const TransformInfo ti = tinfo[blockIdx.y];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = blockIdx.x * 4 + wv;
+    hist[wv][lane] = 0;
+    __syncthreads();
+    if (b < ti.nbins) {
+        const int64_t p0 = binstart[ti.bs0 + b], p1 = binstart[ti.bs0 + b + 1];
+        for (int64_t p = p0 + lane; p < p1; p += 64) {
+            const int sg = min(max(srec[p].lx, 0) >> ti.sxsh, ti.nseg - 1);
+            atomicAdd(&hist[wv][sg], 1);
+        }
+    }
+    __syncthreads();
+    if (b < ti.nbins) {
+        const int ns = ti.nseg;
+        int v = 0;
+        if (lane < ns) v = hist[wv][lane] + hist[wv][(lane + ns - 1) % ns] + hist[wv][(lane + ns - 2) % ns];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+        if (lane == 0) binmax[ti.bs0 + b] = v;
+    }
+}
+
 // strengths of the plan API: cs[tr][k] = c[tr][perm[k]] (conjugated for iflag > 0)
 template <typename R>
 __global__ void k_gather_strengths(int64_t M, int ntr, const int *__restrict__ perm,
@@ -546,6 +580,15 @@ void launch_binstart(const int64_t *Hs, const TransformInfo *tinfo, int ntr, int
 }
 
 template <typename R>
+void launch_binmax(const PtRec<R> *srec, const int64_t *binstart, const TransformInfo *tinfo, int ntr, int max_nbins,
+                   int *binmax, hipStream_t st)
+{
+    if (ntr <= 0 || max_nbins <= 0) return;
+    hipLaunchKernelGGL((k_binmax<R>), dim3((unsigned)((max_nbins + 3) / 4), (unsigned)ntr), dim3(256), 0, st, srec,
+                       binstart, tinfo, binmax);
+}
+
+template <typename R>
 void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::T *c, typename cplx<R>::T *cs,
                    int conj, hipStream_t st)
 {
@@ -605,6 +648,8 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
                                          const PtRec<R> *, const cplx<R>::T *, PtRec<R> *, cplx<R>::T *, int *,       \
                                          hipStream_t);                                                                \
     template void launch_gather<R>(int64_t, int, const int *, const cplx<R>::T *, cplx<R>::T *, int, hipStream_t);    \
+    template void launch_binmax<R>(const PtRec<R> *, const int64_t *, const TransformInfo *, int, int, int *,         \
+                                   hipStream_t);                                                                      \
     template int colfft_tile_elems<R>(bool);                                                                          \
     template size_t colfft_lds<R>(const ColPass &, bool);                                                             \
     template void launch_colfft_jobs<R>(const ColJob<R> *, const int *, int, int, bool, size_t, hipStream_t);

@@ -15,29 +15,33 @@
 //      with LDS atomics into planar accumulator rows (real plane, imaginary
 //      plane) that carry w ghost cells past x = n1, so a stencil never wraps
 //      inside the loop and every cell address is base + constant;
-//   3. folds the ghost cells back onto x < w, converts the tile to complex R
-//      in place, runs the x FFT of its rows in LDS (rows padded by one
-//      element per 16 against bank conflicts; twiddles in LDS), keeps the N1
-//      central frequencies, divides by phi_hat and writes them: the only HBM
-//      write of the pass.
+//   3. folds the ghost cells back onto x < w, runs the x FFT of its rows in
+//      LDS (the first Stockham stage reads the accumulators, rows padded by
+//      one element per 16 against bank conflicts, twiddles in LDS), keeps the
+//      N1 central frequencies, divides by phi_hat and writes them: the only
+//      HBM write of the pass.
 //
 // Accumulation.  On gfx950 an FP32 LDS atomic add costs ~190 cycles per wave
-// instruction (lanes serialised), an FP64 one ~21-26 and a 64-bit integer
-// one ~8-13 (measured).  The FP32 form therefore accumulates in 64-bit
-// fixed point: a contribution v becomes round(v * 2^q) through one FP64
-// fma with the constant 1.5 * 2^52, whose bit pattern is 0x4338000000000000
-// + round(v * 2^q) for |v * 2^q| < 2^51; the low 48 bits of the integer sum of
-// those patterns are the exact two's complement sum of the rounded values
-// (the constant's low 48 bits are zero).  q is chosen per tile so that the
-// tile's candidate count times the largest strength component stays below
-// 2^45, which bounds every cell sum below 2^46: resolution 2^-q relative to
-// that bound (about 1e-11 of the largest cell for the trajectory workloads),
-// and the result does not depend on the order of the atomics.  The plain
-// FP64 form accumulates with FP64 atomics.
-//
-// Tiles are numbered so that each XCD (workgroups are dealt to the 8 XCDs
-// round robin) runs a contiguous range of tiles: neighbouring tiles share
-// most of their candidates, which then come from that XCD's L2.
+// instruction (lanes serialised), an FP64 one ~21-26, a 64-bit integer one
+// ~8-13 and a 32-bit integer one ~7-9 (measured).  The FP32 form therefore
+// accumulates in fixed point, and the result does not depend on the order
+// of the atomics:
+//  * narrow (k_spread_narrow): 32-bit cells, 8 bytes per complex cell, so a
+//    tile holds twice the rows of a 64-bit one.  A tile's scale 2^q is set
+//    from its covering bound C -- the sum over the sort bins it visits of
+//    each bin's bound on how many of its points cover any one cell -- so
+//    that C * max|c| * 2^q < 2^30: every cell sum stays below 2^31 in
+//    magnitude (the 32-bit adds wrap, the final sums are exact) and the
+//    resolution 2^-q is ~ C * max|c| * 2^-30.  Tiles with C > NARROW_MAX
+//    (dense inputs, where 32 bits would lose precision) are queued for the
+//    wide kernel.
+//  * wide (k_spread_wide): 64-bit cells, G rows per pass.  A contribution v
+//    becomes round(v * 2^q) through one FP64 fma with 1.5 * 2^52, whose bit
+//    pattern is 0x4338000000000000 + round(v * 2^q) for |v * 2^q| < 2^51;
+//    the low 48 bits of the integer sum of such patterns are the exact two's
+//    complement sum (the constant's low 48 bits are zero); q keeps the tile's
+//    candidate count times max|c| below 2^45.  The FP64 form runs this
+//    kernel for every tile with FP64 atomics.
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -50,8 +54,11 @@ template <typename R> struct SpreadCfg;
 template <> struct SpreadCfg<float> { static constexpr int E = 8, NT = 512; };
 template <> struct SpreadCfg<double> { static constexpr int E = 8, NT = 512; };
 
-constexpr int MAX_RUNS = 64;    // candidate runs of one tile (two per visited z-bin row)
-constexpr int MAX_TILE_B = 8;   // by, bz <= 8
+constexpr int MAX_RUNS = 64;     // candidate runs of one tile (two per visited z-bin row)
+constexpr int MAX_TILE_B = 16;   // by, bz <= 16
+constexpr int NARROW_MAX = 256;  // largest covering bound of a 32-bit (narrow) FP32 tile
+// counters after the 8 per-XCD tile counters: fallback items queued, taken
+constexpr int CTR_FB_COUNT = 8 * XCD_CTR_STRIDE, CTR_FB_TAKE = 9 * XCD_CTR_STRIDE;
 
 __device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
 __device__ __forceinline__ int posmod(int a, int n)
@@ -92,15 +99,44 @@ __device__ __forceinline__ void overlap(int l, int r0, int b, int w, int n, int 
     }
 }
 
-// LDS layout: accumulator planes [2][rows][RS] of 8-byte cells (RS = n1 + w
-// ghost cells, rounded to even), reused in place for the FFT tile
-// [rows][padded_len(n1)] of complex R; then n1 twiddles.
+// LDS layout of a tile of `rows` rows (RS = n1 + w ghost cells rounded to
+// even, RSF = padded FFT row, RSX = max(RS, RSF)), G = rows of one x-FFT
+// group:
+//   FFT tile      [rows][RSX] complex R, row r at r * RSX (a wide pass uses
+//                 the first G rows);
+//   narrow planes rows in groups of G; group g owns bytes
+//                 [g, g + 1) * G * RSX * 8 and holds its real plane [G][RS]
+//                 then its imaginary plane [G][RS] of int32, inside its own
+//                 FFT rows;
+//   wide planes   one pass of G rows: [2][G][RS] 8-byte cells;
+// then the n1 twiddles.
 __host__ __device__ inline int spread_row_stride(int n1, int w) { return (n1 + w + 1) & ~1; }
+__host__ __device__ inline int spread_row_stride_x(int n1, int w)
+{
+    const int a = spread_row_stride(n1, w), b = padded_len(n1);
+    return a > b ? a : b;
+}
+template <typename R> int spread_max_rows(int n1)
+{
+    const int te = SpreadCfg<R>::E * SpreadCfg<R>::NT;
+    int g = 1;
+    while (2 * g * n1 <= te) g *= 2;
+    return sizeof(R) == 4 ? 2 * g : g;
+}
+template <typename R> int spread_group_rows(int rows, int n1)
+{
+    const int te = SpreadCfg<R>::E * SpreadCfg<R>::NT;
+    int g = 1;
+    while (2 * g <= rows && 2 * g * n1 <= te) g *= 2;
+    return g;
+}
 template <typename R> size_t spread_lds_bytes(int rows, int n1, int w)
 {
-    const size_t accb = (size_t)2 * rows * spread_row_stride(n1, w) * 8;
-    const size_t fftb = (size_t)rows * padded_len(n1) * sizeof(typename cplx<R>::T);
-    return ((accb > fftb ? accb : fftb) + 15) / 16 * 16 + (size_t)n1 * sizeof(typename cplx<R>::T);
+    const size_t G = (size_t)spread_group_rows<R>(rows, n1);
+    const size_t RSX = (size_t)spread_row_stride_x(n1, w);
+    const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * 8 : 0;
+    const size_t wideb = std::max(2 * G * spread_row_stride(n1, w) * 8, G * RSX * sizeof(typename cplx<R>::T));
+    return (std::max(narrowb, wideb) + 15) / 16 * 16 + (size_t)n1 * sizeof(typename cplx<R>::T);
 }
 
 constexpr double FIX_MAGIC = 6755399441055744.0;  // 1.5 * 2^52
@@ -163,6 +199,28 @@ __device__ __forceinline__ void stage_any(int R, int n, int cnt, FDiv dnb, FDiv 
     }
 }
 
+// x FFT of `cnt` rows: accumulators -> padded tile -> ... -> kept frequencies
+template <typename C, int E, int NT, class LoadAcc, class LoadTile, class StoreTile, class StoreOut>
+__device__ __forceinline__ void x_fft(const RadixPlan &rp1, int n1, int cnt, const C *__restrict__ tw, LoadAcc la,
+                                      LoadTile lt, StoreTile st, StoreOut so)
+{
+    int Ns = 1;
+    for (int s = 0; s < rp1.nst; ++s) {
+        const int Rr = rp1.r[s];
+        const bool first = (s == 0), last = (s == rp1.nst - 1);
+        if (first && last)
+            stage_any<C, E, NT>(Rr, n1, cnt, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, la, so, false);
+        else if (first)
+            stage_any<C, E, NT>(Rr, n1, cnt, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, la, st, true);
+        else if (last)
+            stage_any<C, E, NT>(Rr, n1, cnt, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, lt, so, false);
+        else
+            stage_any<C, E, NT>(Rr, n1, cnt, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, lt, st, true);
+        if (!last) __syncthreads();
+        Ns *= Rr;
+    }
+}
+
 // binary search of the job owning tile id (tile0s ascending, tile0s[0] = 0)
 __device__ __forceinline__ int job_of(const int *__restrict__ tile0s, int njobs, int id)
 {
@@ -175,272 +233,518 @@ __device__ __forceinline__ int job_of(const int *__restrict__ tile0s, int njobs,
     return __builtin_amdgcn_readfirstlane(lo);  // uniform: id is the workgroup's tile
 }
 
-// Persistent spread + x FFT over the tiles of a job table.
-//
 // Tile ids: job-major; inside a job, id - tile0 = (tile position) * ntr +
 // transform, so the tiles taken one after another cover every transform of
 // a few neighbouring positions (their candidates overlap and come from the
 // XCD's L2), and the species (3x different point counts) are spread evenly.
-// Workgroup b runs on XCD x = b % 8 (round-robin dispatch of the resident
-// grid) and takes the next tile of its XCD's range [xr[x], xr[x+1]) from the
-// counter wc[x]; the host cuts the ranges at equal estimated cost.  While a
-// tile is spread, wave 0 already holds the next tile's id and loads its
-// candidate runs; while it is transformed, every lane already loads its first
-// candidate of the next tile.
-template <typename R, int E, int NT, int W>
-__global__ __launch_bounds__(NT, 4) void k_spread_xfft(const SpreadJob<R> *__restrict__ jobs,
-                                                       const int *__restrict__ tile0s, int njobs,
-                                                       const int *__restrict__ xr, int *__restrict__ wc)
+template <typename R>
+__device__ __forceinline__ void tile_decode(const SpreadJob<R> &J, int id, int &t, int &y0, int &z0)
 {
-    using C = typename cplx<R>::T;
-    constexpr bool FIX = (sizeof(R) == 4);
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ int64_t rbeg[2][MAX_RUNS];  // candidate runs, double-buffered by tile parity
-    __shared__ int rpre[2][MAX_RUNS + 1];
-    __shared__ float s_amax[2];
-    __shared__ int s_next;
+    const int loc = id - J.tile0;
+    const int ti = (int)fdiv((uint32_t)loc, J.dntr);
+    t = loc - ti * J.ntr;
+    const int by = (J.g.dim >= 2) ? J.g.by : 1, bz = (J.g.dim >= 3) ? J.g.bz : 1;
+    const int tz = (int)fdiv((uint32_t)ti, J.dnty);
+    y0 = (ti - tz * ((J.g.dim >= 2) ? J.g.n2 / by : 1)) * by;
+    z0 = tz * bz;
+}
 
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int xcd = blockIdx.x & 7;
-    const int xbeg = xr[xcd], xend = xr[xcd + 1];
-    unsigned long long *accu = reinterpret_cast<unsigned long long *>(smem_raw);
-    double *accd = reinterpret_cast<double *>(smem_raw);
-    C *tile = reinterpret_cast<C *>(smem_raw);
+// sort bins a tile visits: y bins [ylo, ylo + nyb), z bins [zlo, zlo + nzb) (cyclic)
+__device__ __forceinline__ void tile_bins(const Geo &g, int y0, int z0, int &ylo, int &nyb, int &zlo, int &nzb)
+{
+    const int w = g.w, by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
+    ylo = 0;
+    nyb = 1;
+    zlo = 0;
+    nzb = 1;
+    if (g.dim >= 2) {
+        ylo = floordiv(y0 - (w - 1), g.BY);
+        nyb = floordiv(y0 + by - 1, g.BY) - ylo + 1;
+        if (nyb >= g.nbY) {
+            ylo = 0;
+            nyb = g.nbY;
+        }
+    }
+    if (g.dim >= 3) {
+        zlo = floordiv(z0 - (w - 1), g.BZ);
+        nzb = floordiv(z0 + bz - 1, g.BZ) - zlo + 1;
+        if (nzb >= g.nbZ) {
+            zlo = 0;
+            nzb = g.nbZ;
+        }
+    }
+}
 
-    // geometry of a job's tile: transform t, first row y0, first plane z0
-    auto decode = [&](const SpreadJob<R> &J, int id, int &t, int &y0, int &z0) {
-        const int loc = id - J.tile0;
-        const int ti = (int)fdiv((uint32_t)loc, J.dntr);
-        t = loc - ti * J.ntr;
-        const int by = (J.g.dim >= 2) ? J.g.by : 1, bz = (J.g.dim >= 3) ? J.g.bz : 1;
-        const int tz = (int)fdiv((uint32_t)ti, J.dnty);
-        y0 = (ti - tz * ((J.g.dim >= 2) ? J.g.n2 / by : 1)) * by;
-        z0 = tz * bz;
-    };
-    // number of candidate runs of a tile: two per visited z-bin row
-    auto nruns_of = [&](const SpreadJob<R> &J, int z0) {
-        const Geo &g = J.g;
-        int nzb = 1;
-        if (g.dim >= 3) {
-            const int zlo = floordiv(z0 - (g.w - 1), g.BZ);
-            nzb = floordiv(z0 + g.bz - 1, g.BZ) - zlo + 1;
-            if (nzb >= g.nbZ) nzb = g.nbZ;
-        }
-        return 2 * nzb;
-    };
-    // wave 0: start and size of run `lane` of a tile (registers)
-    auto runs_load = [&](const SpreadJob<R> &J, int t, int y0, int z0, int &nruns, int64_t &b0, int &cnt) {
-        const Geo &g = J.g;
-        const int w = g.w, by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
-        int ylo = 0, nyb = 1, zlo = 0, nzb = 1;
-        if (g.dim >= 2) {
-            ylo = floordiv(y0 - (w - 1), g.BY);
-            nyb = floordiv(y0 + by - 1, g.BY) - ylo + 1;
-            if (nyb >= g.nbY) {
-                ylo = 0;
-                nyb = g.nbY;
-            }
-        }
-        if (g.dim >= 3) {
-            zlo = floordiv(z0 - (w - 1), g.BZ);
-            nzb = floordiv(z0 + bz - 1, g.BZ) - zlo + 1;
-            if (nzb >= g.nbZ) {
-                zlo = 0;
-                nzb = g.nbZ;
-            }
-        }
-        nruns = 2 * nzb;  // <= MAX_RUNS (checked by the host)
-        b0 = 0;
-        cnt = 0;
-        if (lane < nruns) {
-            const int64_t *bs = J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1);
-            const int k = lane >> 1, part = lane & 1;
-            const int zb = posmod(zlo + k, g.nbZ);
-            int ya = 0, yb = 0;  // bin range [ya, yb) of this part
-            const int a = ylo, b = ylo + nyb;
-            if (nyb >= g.nbY) {
-                if (part == 0) yb = g.nbY;
-            } else if (a < 0) {
-                if (part == 0) {
-                    ya = a + g.nbY;
-                    yb = g.nbY;
-                } else {
-                    yb = b;
-                }
-            } else if (b > g.nbY) {
-                if (part == 0) {
-                    ya = a;
-                    yb = g.nbY;
-                } else {
-                    yb = b - g.nbY;
-                }
-            } else if (part == 0) {
-                ya = a;
+__device__ __forceinline__ int nruns_of(const Geo &g, int y0, int z0)
+{
+    int ylo, nyb, zlo, nzb;
+    tile_bins(g, y0, z0, ylo, nyb, zlo, nzb);
+    return 2 * nzb;
+}
+
+// candidate runs, strength bound and covering bound of the current and the
+// next tile (double-buffered), and the next tile's id, in LDS
+struct RunTable {
+    int64_t rbeg[2][MAX_RUNS];
+    int rpre[2][MAX_RUNS + 1];
+    float amax[2];
+    int cb[2];
+    int next;
+};
+
+// Wave 0: the runs of a tile (start and size of run `lane`, inclusive scan of
+// the sizes), its strength bound and, with `bound`, its covering bound (the
+// sum of the visited bins' bounds), published in buffer pb.
+template <typename R>
+__device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJob<R> &J, int t, int y0, int z0,
+                                             int lane, bool bound)
+{
+    const Geo &g = J.g;
+    int ylo, nyb, zlo, nzb;
+    tile_bins(g, y0, z0, ylo, nyb, zlo, nzb);
+    const int nruns = 2 * nzb;  // <= MAX_RUNS (checked by the host)
+    int64_t b0 = 0;
+    int cnt = 0;
+    if (lane < nruns) {
+        const int64_t *bs = J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1);
+        const int k = lane >> 1, part = lane & 1;
+        const int zb = posmod(zlo + k, g.nbZ);
+        int ya = 0, yb = 0;  // bin range [ya, yb) of this part
+        const int a = ylo, b = ylo + nyb;
+        if (nyb >= g.nbY) {
+            if (part == 0) yb = g.nbY;
+        } else if (a < 0) {
+            if (part == 0) {
+                ya = a + g.nbY;
+                yb = g.nbY;
+            } else {
                 yb = b;
             }
-            if (yb > ya) {
-                b0 = bs[zb * g.nbY + ya];
-                cnt = (int)(bs[zb * g.nbY + yb] - b0);
+        } else if (b > g.nbY) {
+            if (part == 0) {
+                ya = a;
+                yb = g.nbY;
+            } else {
+                yb = b - g.nbY;
             }
+        } else if (part == 0) {
+            ya = a;
+            yb = b;
         }
-    };
-    // wave 0: publish the runs (inclusive scan of the sizes) in buffer pb
-    auto runs_store = [&](int pb, int nruns, int64_t b0, int cnt, float am) {
-        int x = cnt;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
+        if (yb > ya) {
+            b0 = bs[zb * g.nbY + ya];
+            cnt = (int)(bs[zb * g.nbY + yb] - b0);
         }
-        if (lane < nruns) {
-            rbeg[pb][lane] = b0;
-            rpre[pb][lane + 1] = x;
-        }
-        if (lane == 0) {
-            rpre[pb][0] = 0;
-            s_amax[pb] = am;
-        }
-    };
-    auto load_tw = [&](const SpreadJob<R> &J) {
-        C *tw = reinterpret_cast<C *>(smem_raw + J.lds_tw);
-        for (int i = threadIdx.x; i < J.g.n1; i += NT) tw[i] = J.tw1[i];
-    };
-
-    // ---- first tile
-    if (threadIdx.x == 0) s_next = xbeg + atomicAdd(wc + xcd, 1);
-    __syncthreads();
-    int cur = s_next;
-    if (cur >= xend) return;
-    int j = job_of(tile0s, njobs, cur);
-    int t, y0, z0;
-    decode(jobs[j], cur, t, y0, z0);
-    load_tw(jobs[j]);
-    int pb = 0;  // run buffer of the current tile
-    if (wv == 0) {
-        int nr, cnt;
-        int64_t b0;
-        runs_load(jobs[j], t, y0, z0, nr, b0, cnt);
-        runs_store(pb, nr, b0, cnt, jobs[j].amax ? jobs[j].amax[t] : 1.0f);
     }
-    __syncthreads();
+    int cb = 0x7fffffff;
+    if (bound && J.binmax) {
+        const int *bm = J.binmax + (int64_t)t * J.bin_tstride * (g.nbins + 1);
+        const int nv = nyb * nzb;
+        int sum = 0;
+        for (int i = lane; i < nv; i += 64) {
+            const int k = i / nyb, jy = i - k * nyb;
+            sum += bm[posmod(zlo + k, g.nbZ) * g.nbY + posmod(ylo + jy, g.nbY)];
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+        cb = sum;
+    }
+    int x = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane < nruns) {
+        S.rbeg[pb][lane] = b0;
+        S.rpre[pb][lane + 1] = x;
+    }
+    if (lane == 0) {
+        S.rpre[pb][0] = 0;
+        S.amax[pb] = J.amax ? J.amax[t] : 1.0f;
+        S.cb[pb] = cb;
+    }
+}
 
-    // candidate fetch from the runs currently published
+// one lane's candidate (sorted record and strength): candidate v of the
+// tile whose runs are in buffer b, with e the lane's run cursor
+template <typename R> struct Cand {
     PtRec<R> pr;
-    C cv;
-    bool have = false;
-    int e = 0, total = 0;
-    auto fetch = [&](const SpreadJob<R> &J, int b, int v, int tt) {
+    typename cplx<R>::T cv;
+    bool have;
+    int e;
+    __device__ __forceinline__ void fetch(const SpreadJob<R> &J, const RunTable &S, int b, int v, int tt, int total)
+    {
         have = v < total;
         if (have) {
-            while (rpre[b][e + 1] <= v) ++e;
-            const int64_t p = rbeg[b][e] + (v - rpre[b][e]);
+            while (S.rpre[b][e + 1] <= v) ++e;
+            const int64_t p = S.rbeg[b][e] + (v - S.rpre[b][e]);
             pr = J.srec[p];
             cv = J.cs[(int64_t)tt * J.c_tstride + p];
         }
-    };
-    total = rpre[pb][nruns_of(jobs[j], z0)];
-    fetch(jobs[j], pb, wv * 64 + lane, t);
+    }
+};
+
+// kept frequency f of tile row r (deconvolved, conjugated if asked) to G1
+template <typename R>
+__device__ __forceinline__ void store_kept(const SpreadJob<R> &J, int t, int y0, int z0, int by, int r, int f,
+                                           typename cplx<R>::T v)
+{
+    const Geo &g = J.g;
+    const int N1 = g.N1, h1 = N1 / 2, n1 = g.n1;
+    int kk;
+    if (f < N1 - h1) kk = f + h1;
+    else if (f >= n1 - h1) kk = f - (n1 - h1);
+    else return;
+    v = cscale(v, J.phinv1[kk]);
+    if (J.final_conj) v.y = -v.y;
+    const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
+    J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
+}
+
+template <typename R, int NT> __device__ __forceinline__ void load_twiddles(const SpreadJob<R> &J, unsigned char *smem)
+{
+    using C = typename cplx<R>::T;
+    C *tw = reinterpret_cast<C *>(smem + J.lds_tw);
+    for (int i = threadIdx.x; i < J.g.n1; i += NT) tw[i] = J.tw1[i];
+}
+
+// ---------------------------------------------------------------------------
+// Narrow (32-bit) FP32 spread + x FFT, persistent over the tiles of a job
+// table.  Workgroup b runs on XCD x = b % 8 (round-robin dispatch of the
+// resident grid) and takes the next tile of its XCD's range [xr[x], xr[x+1])
+// from the counter wc[x * XCD_CTR_STRIDE]; the host cuts the ranges at equal
+// estimated cost.  While a tile is spread, wave 0 already holds the next
+// tile's id and loads its runs and bounds; while it is transformed, every
+// lane already loads its first candidate of the next tile.  A tile whose
+// covering bound exceeds NARROW_MAX (or whose strengths are not finite) is
+// appended to fbl as its wide passes (tile id * 2 + pass) instead.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float> *__restrict__ jobs,
+                                                          const int *__restrict__ tile0s, int njobs,
+                                                          const int *__restrict__ xr, int *__restrict__ wc,
+                                                          int *__restrict__ fbl)
+{
+    using R = float;
+    using C = float2;
+    constexpr int E = SpreadCfg<float>::E, NT = SpreadCfg<float>::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ RunTable S;
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // first candidate of this wave: the last wave takes the first 64, so that
+    // wave 0, which also loads the next tile's runs, gets the ragged tail
+    const int vw = (NT / 64 - 1 - wv) * 64;
+    const int xcd = blockIdx.x & 7;
+    const int xbeg = xr[xcd], xend = xr[xcd + 1];
+    int *acc = reinterpret_cast<int *>(smem_raw);
+    C *tile = reinterpret_cast<C *>(smem_raw);
+
+    if (threadIdx.x == 0) S.next = xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
+    __syncthreads();
+    int cur = S.next;
+    if (cur >= xend) return;
+    int j = job_of(tile0s, njobs, cur);
+    int t, y0, z0;
+    tile_decode(jobs[j], cur, t, y0, z0);
+    load_twiddles<R, NT>(jobs[j], smem_raw);
+    int pb = 0;
+    if (wv == 0) runs_publish(S, pb, jobs[j], t, y0, z0, lane, true);
+    __syncthreads();
+    Cand<R> cd;
+    cd.have = false;
+    cd.e = 0;
+    int total = S.rpre[pb][nruns_of(jobs[j].g, y0, z0)];
+    cd.fetch(jobs[j], S, pb, vw + lane, t, total);
 
     for (;;) {
         const SpreadJob<R> &J = jobs[j];
         const Geo &g = J.g;
         const int n1 = g.n1, w = g.w;
         const int RS = spread_row_stride(n1, W);
-        const int RSF = padded_len(n1);
+        const int RSX = spread_row_stride_x(n1, W);
         const int by = (g.dim >= 2) ? g.by : 1;
         const int bz = (g.dim >= 3) ? g.bz : 1;
         const int rows = by * bz;
-        const int plane = rows * RS;
+        const int G = J.grp, gsh = J.grp_sh;
+        const int GS32 = G * RSX * 2;  // int32 cells of one group's region
         const R beta = (R)g.beta, ic = (R)g.ic;
         const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
+        const float am = S.amax[pb];
+        const int cb = S.cb[pb];
+        const bool ok = (am < 3.0e38f) && cb <= NARROW_MAX;
+        int q = 0;
+        if (ok && am > 0.0f) q = 29 - ilogb((double)max(cb, 1) * (double)am);
 
-        // ---- zero the accumulator planes; take the next tile's id
-        {
+        if (ok) {
             uint4 *a4 = reinterpret_cast<uint4 *>(smem_raw);
-            for (int i = threadIdx.x; i < plane; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+            for (int i = threadIdx.x; i < rows * RSX / 2; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+        } else if (threadIdx.x == 0) {
+            const int np = rows >> gsh;
+            const int k = atomicAdd(wc + CTR_FB_COUNT, np);
+            for (int p = 0; p < np; ++p) fbl[k + p] = cur * 2 + p;
         }
-        const float am = s_amax[pb];
         __syncthreads();
         if (wv == 0) {
-            // the next tile's id and runs (other buffer) while this one is spread
+            // the next tile's id, runs and bounds (other buffer) while this one is spread
             int nxt = 0;
-            if (lane == 0) nxt = xbeg + atomicAdd(wc + xcd, 1);
+            if (lane == 0) nxt = xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
             nxt = __shfl(nxt, 0, 64);
-            if (lane == 0) s_next = nxt;
+            if (lane == 0) S.next = nxt;
             if (nxt < xend) {
-                int jn, tn, y0n, z0n, nr, cnt;
-                int64_t b0;
-                jn = job_of(tile0s, njobs, nxt);
-                decode(jobs[jn], nxt, tn, y0n, z0n);
-                runs_load(jobs[jn], tn, y0n, z0n, nr, b0, cnt);
-                runs_store(pb ^ 1, nr, b0, cnt, jobs[jn].amax ? jobs[jn].amax[tn] : 1.0f);
+                int jq, tq, y0q, z0q;
+                jq = job_of(tile0s, njobs, nxt);
+                tile_decode(jobs[jq], nxt, tq, y0q, z0q);
+                runs_publish(S, pb ^ 1, jobs[jq], tq, y0q, z0q, lane, true);
             }
         }
 
-        // fixed-point scale 2^q: total * amax * 2^q < 2^45
-        double fscale = 1.0, finv = 1.0;
-        bool fbad = false;
-        if (FIX) {
-            fbad = !(am < 3.0e38f);
-            const double bound = (double)max(total, 1) * (double)(am > 0.0f && !fbad ? am : 1.0f);
-            const int q = 44 - ilogb(bound);
-            fscale = ldexp(1.0, q);
-            finv = ldexp(1.0, -q);
+        // ---- spread: every wave walks candidates v = vw + lane + k*NT
+        if (ok) {
+            for (int v0 = vw; v0 < total; v0 += NT) {
+                const PtRec<R> p = cd.pr;
+                const C cc = cd.cv;
+                const bool hv = cd.have;
+                cd.fetch(J, S, pb, v0 + NT + lane, t, total);
+                if (!hv) continue;
+                int fy = 0, ny = 1, my = 0, fz = 0, nz = 1, mz = 0;
+                if (g.dim >= 2) overlap(p.ly, y0, by, w, g.n2, fy, ny, my);
+                if (g.dim >= 3) overlap(p.lz, z0, bz, w, g.n3, fz, nz, mz);
+                if (ny <= 0 || nz <= 0) continue;
+                R kx[W];
+#pragma unroll
+                for (int m = 0; m < W; ++m) kx[m] = es_kernel(p.dx + (R)m, beta, ic);
+                R ky[W];
+#pragma unroll
+                for (int iy = 0; iy < W; ++iy)
+                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
+                for (int iz = 0; iz < nz; ++iz) {
+                    const R wz = (g.dim >= 3) ? es_kernel(p.dz + (R)(mz + iz), beta, ic) : (R)1;
+                    const int rr0 = (fz + iz) * by + fy;  // tile row of iy = 0
+#pragma unroll
+                    for (int iy = 0; iy < W; ++iy) {
+                        if (iy < ny) {
+                            const int r = rr0 + iy;
+                            const R wyz = ky[iy] * wz;
+                            const int c0 = (r >> gsh) * GS32 + (r & (G - 1)) * RS + p.lx;
+                            const float sr = ldexpf(cc.x * wyz, q), si = ldexpf(cc.y * wyz, q);
+                            int *ar = acc + c0, *ai = acc + c0 + G * RS;
+#pragma unroll
+                            for (int m = 0; m < W; ++m) {
+                                __hip_atomic_fetch_add(ar + m, __float2int_rn(sr * kx[m]), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                                __hip_atomic_fetch_add(ai + m, __float2int_rn(si * kx[m]), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- fold the ghost cells (32-bit adds wrap; the final sums are exact)
+        if (ok) {
+            for (int i = threadIdx.x; i < 2 * rows * W; i += NT) {
+                const int pl = i / (rows * W), rr = (i / W) % rows, x = i % W;
+                const int a = (rr >> gsh) * GS32 + pl * G * RS + (rr & (G - 1)) * RS;
+                acc[a + x] += acc[a + n1 + x];
+            }
+        }
+        __syncthreads();
+        // the next tile's first candidates are in flight during this FFT
+        const int nxt = S.next;
+        const bool has_next = nxt < xend;
+        int jn = j, tn = 0, y0n = 0, z0n = 0, total_n = 0;
+        if (has_next) {
+            jn = job_of(tile0s, njobs, nxt);
+            tile_decode(jobs[jn], nxt, tn, y0n, z0n);
+            total_n = S.rpre[pb ^ 1][nruns_of(jobs[jn].g, y0n, z0n)];
+            cd.e = 0;
+            cd.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n);
         }
 
-        // ---- spread: every wave walks candidates v = wv*64 + lane + k*NT
-        for (int v0 = wv * 64; v0 < total; v0 += NT) {
-            const PtRec<R> cur_p = pr;
-            const C cc = cv;
-            const bool ok = have;
-            fetch(J, pb, v0 + NT + lane, t);
-            if (!ok) continue;
+        // ---- x FFT, one group of G rows at a time
+        if (ok) {
+            auto group = [&](const int gi) __attribute__((always_inline)) {
+                C *gt = tile + gi * G * RSX;
+                const int *ga = acc + gi * GS32;
+                auto load_acc = [&](int row, int i) -> C {
+                    const int a = row * RS + i;
+                    return cmk<C>(ldexpf((float)ga[a], -q), ldexpf((float)ga[a + G * RS], -q));
+                };
+                auto load_tile = [&](int row, int i) -> C { return gt[row * RSX + padx<true>(i)]; };
+                auto store_tile = [&](int row, int i, C v) { gt[row * RSX + padx<true>(i)] = v; };
+                auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, gi * G + row, f, v); };
+                x_fft<C, E, NT>(J.rp1, n1, G, tw, load_acc, load_tile, store_tile, store_out);
+            };
+            group(0);
+            if (rows > G) group(1);
+        }
+        if (!has_next) break;
+        __syncthreads();  // every read of the tile (and the twiddles) done
+        if (jn != j) load_twiddles<R, NT>(jobs[jn], smem_raw);
+        cur = nxt;
+        pb ^= 1;
+        j = jn;
+        t = tn;
+        y0 = y0n;
+        z0 = z0n;
+        total = total_n;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wide spread + x FFT: one pass of G rows of a tile per item.  Items are
+// either every tile of the table (fbl == nullptr; per-XCD ranges as in the
+// narrow kernel) or the list fbl[0 .. wc[CTR_FB_COUNT]) the narrow kernel
+// queued (item = tile id * 2 + pass), taken through wc[CTR_FB_TAKE].
+// ---------------------------------------------------------------------------
+template <typename R, int W>
+__global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__restrict__ jobs,
+                                                        const int *__restrict__ tile0s, int njobs,
+                                                        const int *__restrict__ xr, int *__restrict__ wc,
+                                                        const int *__restrict__ fbl)
+{
+    using C = typename cplx<R>::T;
+    constexpr bool F32 = (sizeof(R) == 4);
+    constexpr int E = SpreadCfg<R>::E, NT = SpreadCfg<R>::NT;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ RunTable S;
+
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int vw = (NT / 64 - 1 - wv) * 64;
+    const int xcd = blockIdx.x & 7;
+    const int xbeg = fbl ? 0 : xr[xcd], xend = fbl ? wc[CTR_FB_COUNT] : xr[xcd + 1];
+    unsigned long long *acc64 = reinterpret_cast<unsigned long long *>(smem_raw);
+    double *accd = reinterpret_cast<double *>(smem_raw);
+    C *tile = reinterpret_cast<C *>(smem_raw);
+    // next item: its position k in [xbeg, xend) (k >= xend: none) -> tile id, pass
+    auto take = [&]() -> int {
+        return fbl ? atomicAdd(wc + CTR_FB_TAKE, 1) : xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
+    };
+    auto item_of = [&](int k, int &id, int &pass) {
+        int it = fbl ? fbl[k] : 2 * k;
+        it = __builtin_amdgcn_readfirstlane(it);
+        id = it >> 1;
+        pass = it & 1;
+    };
+
+    if (threadIdx.x == 0) S.next = take();
+    __syncthreads();
+    const int kfirst = S.next;
+    if (kfirst >= xend) return;
+    int cur, pass;
+    item_of(kfirst, cur, pass);
+    int j = job_of(tile0s, njobs, cur);
+    int t, y0, z0;
+    tile_decode(jobs[j], cur, t, y0, z0);
+    load_twiddles<R, NT>(jobs[j], smem_raw);
+    int pb = 0;
+    if (wv == 0) runs_publish(S, pb, jobs[j], t, y0, z0, lane, false);
+    __syncthreads();
+    Cand<R> cd;
+    cd.have = false;
+    cd.e = 0;
+    int total = S.rpre[pb][nruns_of(jobs[j].g, y0, z0)];
+    cd.fetch(jobs[j], S, pb, vw + lane, t, total);
+
+    for (;;) {
+        const SpreadJob<R> &J = jobs[j];
+        const Geo &g = J.g;
+        const int n1 = g.n1, w = g.w;
+        const int RS = spread_row_stride(n1, W);
+        const int RSX = spread_row_stride_x(n1, W);
+        const int by = (g.dim >= 2) ? g.by : 1;
+        const int bz = (g.dim >= 3) ? g.bz : 1;
+        const int G = J.grp;
+        const int r0 = pass * G;    // tile rows [r0, r0 + G) of this item
+        const int wplane = G * RS;  // cells of one plane
+        const R beta = (R)g.beta, ic = (R)g.ic;
+        const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
+        const float am = S.amax[pb];
+
+        {
+            uint4 *a4 = reinterpret_cast<uint4 *>(smem_raw);
+            for (int i = threadIdx.x; i < wplane; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        if (wv == 0) {
+            int kn = 0;
+            if (lane == 0) kn = take();
+            kn = __shfl(kn, 0, 64);
+            if (lane == 0) S.next = kn;
+            if (kn < xend) {
+                int idq, pq, jq, tq, y0q, z0q;
+                item_of(kn, idq, pq);
+                jq = job_of(tile0s, njobs, idq);
+                tile_decode(jobs[jq], idq, tq, y0q, z0q);
+                runs_publish(S, pb ^ 1, jobs[jq], tq, y0q, z0q, lane, false);
+            }
+        }
+
+        // fixed-point scale 2^q (FP32): total * amax * 2^q < 2^45
+        double fscale = 1.0, finv = 1.0;
+        bool fbad = false;
+        if (F32) {
+            fbad = !(am < 3.0e38f);
+            const double bound = (double)max(total, 1) * (double)(am > 0.0f && !fbad ? am : 1.0f);
+            const int qq = 44 - ilogb(bound);
+            fscale = ldexp(1.0, qq);
+            finv = ldexp(1.0, -qq);
+        }
+
+        for (int v0 = vw; v0 < total; v0 += NT) {
+            const PtRec<R> p = cd.pr;
+            const C cc = cd.cv;
+            const bool hv = cd.have;
+            cd.fetch(J, S, pb, v0 + NT + lane, t, total);
+            if (!hv) continue;
             int fy = 0, ny = 1, my = 0, fz = 0, nz = 1, mz = 0;
-            if (g.dim >= 2) overlap(cur_p.ly, y0, by, w, g.n2, fy, ny, my);
-            if (g.dim >= 3) overlap(cur_p.lz, z0, bz, w, g.n3, fz, nz, mz);
+            if (g.dim >= 2) overlap(p.ly, y0, by, w, g.n2, fy, ny, my);
+            if (g.dim >= 3) overlap(p.lz, z0, bz, w, g.n3, fz, nz, mz);
             if (ny <= 0 || nz <= 0) continue;
             double kxd[W];
             R kx[W];
 #pragma unroll
             for (int m = 0; m < W; ++m) {
-                kx[m] = es_kernel(cur_p.dx + (R)m, beta, ic);
+                kx[m] = es_kernel(p.dx + (R)m, beta, ic);
                 kxd[m] = (double)kx[m] * fscale;
             }
-            R ky[MAX_TILE_B];
+            R ky[W];
 #pragma unroll
-            for (int iy = 0; iy < MAX_TILE_B; ++iy) {
-                ky[iy] = (R)0;
-                if (iy < by)
-                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(cur_p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
-            }
+            for (int iy = 0; iy < W; ++iy)
+                ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
             for (int iz = 0; iz < nz; ++iz) {
-                const R wz = (g.dim >= 3) ? es_kernel(cur_p.dz + (R)(mz + iz), beta, ic) : (R)1;
-                const int rbase = ((fz + iz) * by + fy) * RS + cur_p.lx;
+                const R wz = (g.dim >= 3) ? es_kernel(p.dz + (R)(mz + iz), beta, ic) : (R)1;
+                const int rr0 = (fz + iz) * by + fy - r0;  // pass row of iy = 0
 #pragma unroll
-                for (int iy = 0; iy < MAX_TILE_B; ++iy) {
-                    if (iy < by && iy < ny) {
+                for (int iy = 0; iy < W; ++iy) {
+                    const int r = rr0 + iy;
+                    if (iy < ny && r >= 0 && r < G) {
                         const R wyz = ky[iy] * wz;
-                        const int cell0 = rbase + iy * RS;
-                        if (FIX) {
+                        const int c0 = r * RS + p.lx;
+                        if (F32) {
                             const double vr = (double)(cc.x * wyz), vi = (double)(cc.y * wyz);
-                            unsigned long long *pr_ = accu + cell0, *pi_ = accu + plane + cell0;
+                            unsigned long long *ar = acc64 + c0, *ai = acc64 + wplane + c0;
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
                                 const double dr = __builtin_fma(vr, kxd[m], FIX_MAGIC);
                                 const double di = __builtin_fma(vi, kxd[m], FIX_MAGIC);
-                                __hip_atomic_fetch_add(pr_ + m, (unsigned long long)__double_as_longlong(dr),
+                                __hip_atomic_fetch_add(ar + m, (unsigned long long)__double_as_longlong(dr),
                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_fetch_add(pi_ + m, (unsigned long long)__double_as_longlong(di),
+                                __hip_atomic_fetch_add(ai + m, (unsigned long long)__double_as_longlong(di),
                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         } else {
                             const R vr = cc.x * wyz, vi = cc.y * wyz;
-                            double *pr_ = accd + cell0, *pi_ = accd + plane + cell0;
+                            double *ar = accd + c0, *ai = accd + wplane + c0;
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
-                                __hip_atomic_fetch_add(pr_ + m, (double)(vr * kx[m]), __ATOMIC_RELAXED,
+                                __hip_atomic_fetch_add(ar + m, (double)(vr * kx[m]), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_fetch_add(pi_ + m, (double)(vi * kx[m]), __ATOMIC_RELAXED,
+                                __hip_atomic_fetch_add(ai + m, (double)(vi * kx[m]), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
@@ -451,83 +755,51 @@ __global__ __launch_bounds__(NT, 4) void k_spread_xfft(const SpreadJob<R> *__res
         __syncthreads();
 
         // ---- fold the ghost cells
-        for (int i = threadIdx.x; i < 2 * rows * W; i += NT) {
-            const int pl = i / (rows * W), rr = (i / W) % rows, x = i % W;
-            const int a = pl * plane + rr * RS;
-            if (FIX) accu[a + x] += accu[a + n1 + x];
+        for (int i = threadIdx.x; i < 2 * G * W; i += NT) {
+            const int pl = i / (G * W), rr = (i / W) % G, x = i % W;
+            const int a = pl * wplane + rr * RS;
+            if (F32) acc64[a + x] += acc64[a + n1 + x];
             else accd[a + x] += accd[a + n1 + x];
         }
         __syncthreads();
-        // the next tile's first candidates are in flight during this FFT
-        const int nxt = s_next;
-        const bool has_next = nxt < xend;
-        int jn = j, tn = 0, y0n = 0, z0n = 0, total_n = 0;
+        const int kn = S.next;
+        const bool has_next = kn < xend;
+        int nid = 0, npass = 0, jn = j, tn = 0, y0n = 0, z0n = 0, total_n = 0;
         if (has_next) {
-            jn = job_of(tile0s, njobs, nxt);
-            decode(jobs[jn], nxt, tn, y0n, z0n);
-            total_n = rpre[pb ^ 1][nruns_of(jobs[jn], z0n)];
-            const int tot_cur = total;
-            total = total_n;
-            e = 0;
-            fetch(jobs[jn], pb ^ 1, wv * 64 + lane, tn);
-            total = tot_cur;
+            item_of(kn, nid, npass);
+            jn = job_of(tile0s, njobs, nid);
+            tile_decode(jobs[jn], nid, tn, y0n, z0n);
+            total_n = S.rpre[pb ^ 1][nruns_of(jobs[jn].g, y0n, z0n)];
+            cd.e = 0;
+            cd.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n);
         }
 
-        // ---- x FFT: accumulator -> padded tile -> ... -> kept frequencies
-        const int N1 = g.N1, h1 = N1 / 2;
-        auto load_acc = [&](int row, int i) -> C {
-            const int a = row * RS + i;
-            if (FIX) {
-                const long long br = (long long)(accu[a] << 16) >> 16;
-                const long long bi = (long long)(accu[plane + a] << 16) >> 16;
-                return cmk<C>(fbad ? (R)NAN : (R)((double)br * finv), fbad ? (R)NAN : (R)((double)bi * finv));
-            } else {
-                return cmk<C>((R)accd[a], (R)accd[plane + a]);
-            }
-        };
-        auto load_tile = [&](int row, int i) -> C { return tile[row * RSF + padx<true>(i)]; };
-        auto store_tile = [&](int row, int i, C v) { tile[row * RSF + padx<true>(i)] = v; };
-        auto store_out = [&](int row, int f, C v) {
-            int kk;
-            if (f < N1 - h1) kk = f + h1;
-            else if (f >= n1 - h1) kk = f - (n1 - h1);
-            else return;
-            v = cscale(v, J.phinv1[kk]);
-            if (J.final_conj) v.y = -v.y;
-            const int rz = (int)fdiv((uint32_t)row, J.dby), ry = row - rz * by;
-            J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
-        };
+        // ---- x FFT of the item's G rows
         {
-            const RadixPlan &rp1 = J.rp1;
-            int Ns = 1;
-            for (int s = 0; s < rp1.nst; ++s) {
-                const int Rr = rp1.r[s];
-                const bool first = (s == 0), last = (s == rp1.nst - 1);
-                if (first && last)
-                    stage_any<C, E, NT>(Rr, n1, rows, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, load_acc, store_out,
-                                        false);
-                else if (first)
-                    stage_any<C, E, NT>(Rr, n1, rows, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, load_acc,
-                                        store_tile, true);
-                else if (last)
-                    stage_any<C, E, NT>(Rr, n1, rows, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, load_tile,
-                                        store_out, false);
-                else
-                    stage_any<C, E, NT>(Rr, n1, rows, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, load_tile,
-                                        store_tile, true);
-                if (!last) __syncthreads();
-                Ns *= Rr;
-            }
+            auto load_acc = [&](int row, int i) -> C {
+                const int a = row * RS + i;
+                if (F32) {
+                    const long long br = (long long)(acc64[a] << 16) >> 16;
+                    const long long bi = (long long)(acc64[wplane + a] << 16) >> 16;
+                    return cmk<C>(fbad ? (R)NAN : (R)((double)br * finv), fbad ? (R)NAN : (R)((double)bi * finv));
+                } else {
+                    return cmk<C>((R)accd[a], (R)accd[wplane + a]);
+                }
+            };
+            auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
+            auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
+            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, r0 + row, f, v); };
+            x_fft<C, E, NT>(J.rp1, n1, G, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
         __syncthreads();  // every read of the tile (and the twiddles) done
-        if (jn != j) load_tw(jobs[jn]);
-        cur = nxt;
+        if (jn != j) load_twiddles<R, NT>(jobs[jn], smem_raw);
         pb ^= 1;
         j = jn;
         t = tn;
         y0 = y0n;
         z0 = z0n;
+        pass = npass;
         total = total_n;
     }
 }
@@ -548,42 +820,55 @@ void launch_zero_i32(int *p, int n, hipStream_t st)
     if (n > 0) hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
 }
 
-template <typename R, int W>
-static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, const int *xr, int *wc, size_t lds,
-                     hipStream_t st)
+// workgroups of a persistent launch: CUs x resident workgroups per CU at the
+// launch's LDS size (cached per kernel and size), rounded to the 8 XCDs
+static int persistent_grid(const void *fn, int nt, size_t lds)
 {
-    constexpr int E = SpreadCfg<R>::E, NT = SpreadCfg<R>::NT;
-    const void *fn = reinterpret_cast<const void *>(&k_spread_xfft<R, E, NT, W>);
     static std::mutex mu;
-    static std::map<size_t, int> resident_at;  // workgroups per CU at an LDS size
-    static int ncu = 0;
-    int resident;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        auto it = resident_at.find(lds);
-        if (it == resident_at.end()) {
-            (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
-            int dev = 0, nb = 0;
-            (void)hipGetDevice(&dev);
-            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, NT, lds) != hipSuccess || nb < 1) nb = 1;
-            it = resident_at.emplace(lds, nb).first;
-        }
-        resident = it->second;
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(fn, lds);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
+        int dev = 0, nb = 0, ncu = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, nt, lds) != hipSuccess || nb < 1) nb = 1;
+        const int grid = std::max(1, ncu) * nb;
+        it = cache.emplace(key, (grid + 7) / 8 * 8).first;
     }
-    int grid = std::max(1, ncu) * resident;
-    grid = ((grid + 7) / 8) * 8;
-    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, wc, 8);
-    hipLaunchKernelGGL((k_spread_xfft<R, E, NT, W>), dim3(grid), dim3(NT), lds, st, jobs, tile0s, njobs, xr, wc);
+    return it->second;
+}
+
+template <typename R, int W>
+static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, const int *xr, int *wc, int *fbl,
+                     size_t lds, hipStream_t st)
+{
+    constexpr int NT = SpreadCfg<R>::NT;
+    launch_zero_i32(wc, XCD_CTR_INTS, st);
+    const void *fw = reinterpret_cast<const void *>(&k_spread_wide<R, W>);
+    if constexpr (sizeof(R) == 4) {
+        const void *fn = reinterpret_cast<const void *>(&k_spread_narrow<W>);
+        hipLaunchKernelGGL((k_spread_narrow<W>), dim3((unsigned)persistent_grid(fn, NT, lds)), dim3(NT), lds, st, jobs,
+                           tile0s, njobs, xr, wc, fbl);
+        // the tiles the narrow kernel queued (usually none: its workgroups then
+        // only read the empty count)
+        hipLaunchKernelGGL((k_spread_wide<R, W>), dim3((unsigned)persistent_grid(fw, NT, lds)), dim3(NT), lds, st,
+                           jobs, tile0s, njobs, xr, wc, (const int *)fbl);
+    } else {
+        hipLaunchKernelGGL((k_spread_wide<R, W>), dim3((unsigned)persistent_grid(fw, NT, lds)), dim3(NT), lds, st,
+                           jobs, tile0s, njobs, xr, wc, (const int *)nullptr);
+    }
 }
 
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
-                        size_t lds, hipStream_t st)
+                        int *fbl, size_t lds, hipStream_t st)
 {
     if (njobs <= 0) return;
 #define MPSQ_SPREAD_W(WV)                                                                                             \
-    case WV: launch_w<R, WV>(jobs, tile0s, njobs, xr, wc, lds, st); break;
+    case WV: launch_w<R, WV>(jobs, tile0s, njobs, xr, wc, fbl, lds, st); break;
     switch (w) {
         MPSQ_SPREAD_W(2)
         MPSQ_SPREAD_W(3)
@@ -607,9 +892,11 @@ void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, 
 
 #define MPSQ_SPREAD_INST(R)                                                                                           \
     template int spread_tile_elems<R>();                                                                              \
+    template int spread_max_rows<R>(int);                                                                             \
+    template int spread_group_rows<R>(int, int);                                                                      \
     template size_t spread_lds<R>(int, int, int);                                                                     \
-    template void launch_spread_jobs<R>(const SpreadJob<R> *, const int *, int, int, const int *, int *, size_t,     \
-                                        hipStream_t);
+    template void launch_spread_jobs<R>(const SpreadJob<R> *, const int *, int, int, const int *, int *, int *,      \
+                                        size_t, hipStream_t);
 
 MPSQ_SPREAD_INST(float)
 MPSQ_SPREAD_INST(double)

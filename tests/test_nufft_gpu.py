@@ -52,6 +52,44 @@ def test_type1_vs_direct_dft(gpu_lib, dim, nm, dtype, eps, isign):
     assert err < _tol(dtype, eps), f"rel l2 {err:.3e}"
 
 
+@pytest.mark.parametrize("case", ["dense2d", "clustered3d", "sparse3d"])
+def test_fp32_accumulation_modes(gpu_lib, case):
+    """FP32 plans spread each tile in 32-bit fixed point when the tile's
+    covering bound allows it, else in 64-bit fixed point (a second launch).
+    dense2d: every tile takes the 64-bit path; clustered3d: 20000 of 22000
+    points in a small ball, so the tiles through the ball (~13%) take the
+    64-bit path and the others the 32-bit one in the same call; sparse3d:
+    every tile 32-bit.
+    Each is checked against the direct DFT and against the FP64 plan."""
+    from mp_tools_amd import nufft
+    rng = np.random.default_rng(zlib.crc32(case.encode()))
+    if case == "dense2d":
+        nm, M = (32, 28), 120000
+        co = _pts(rng, M, 2)
+    elif case == "clustered3d":
+        nm, M = (40, 36, 24), 22000
+        co = _pts(rng, M, 3)
+        k = 20000
+        for d in range(3):
+            co[d][:k] = rng.normal(0.5, 0.15, k)
+    else:
+        nm, M = (40, 36, 24), 2000
+        co = _pts(rng, M, 3)
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    ref = O.dirft_type1(*(co + [None] * (3 - len(nm))), c, nm)
+    p32 = nufft.Plan(1, nm, eps=1e-6, dtype="complex64")
+    p32.setpts(*co)
+    f32 = p32.execute(c)
+    p64 = nufft.Plan(1, nm, eps=1e-6, dtype="complex128")
+    p64.setpts(*co)
+    f64 = p64.execute(c)
+    e32, e64 = rel_l2(f32, ref), rel_l2(f64, ref)
+    print(f"{case}: FP32 {e32:.3e}, FP64 {e64:.3e} vs direct DFT")
+    assert e64 < 1e-5
+    assert e32 < 2e-5
+    assert rel_l2(f32, f64) < 2e-5
+
+
 @pytest.mark.parametrize("dtype", ["complex128", "complex64"])
 @pytest.mark.parametrize("dim,nm", [(2, (40, 36)), (3, (24, 20, 12))])
 def test_sort_permutation_bit_exact(gpu_lib, dtype, dim, nm):
