@@ -367,13 +367,14 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
 }
 
 // one lane's candidate (sorted record and strength): candidate v of the
-// tile whose runs are in buffer b, with e the lane's run cursor
+// tile whose runs are in buffer b; e is the lane's run cursor (v ascending
+// between resets)
 template <typename R> struct Cand {
     PtRec<R> pr;
     typename cplx<R>::T cv;
     bool have;
-    int e;
-    __device__ __forceinline__ void fetch(const SpreadJob<R> &J, const RunTable &S, int b, int v, int tt, int total)
+    __device__ __forceinline__ void fetch(const SpreadJob<R> &J, const RunTable &S, int b, int v, int tt, int total,
+                                          int &e)
     {
         have = v < total;
         if (have) {
@@ -452,11 +453,12 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
     int pb = 0;
     if (wv == 0) runs_publish(S, pb, jobs[j], t, y0, z0, lane, true);
     __syncthreads();
-    Cand<R> cd;
-    cd.have = false;
-    cd.e = 0;
+    // two candidates per lane in flight: this one and the one NT further
+    Cand<R> c0, c1;
+    int e = 0;
     int total = S.rpre[pb][nruns_of(jobs[j].g, y0, z0)];
-    cd.fetch(jobs[j], S, pb, vw + lane, t, total);
+    c0.fetch(jobs[j], S, pb, vw + lane, t, total, e);
+    c1.fetch(jobs[j], S, pb, vw + NT + lane, t, total, e);
 
     for (;;) {
         const SpreadJob<R> &J = jobs[j];
@@ -503,10 +505,11 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
         // ---- spread: every wave walks candidates v = vw + lane + k*NT
         if (ok) {
             for (int v0 = vw; v0 < total; v0 += NT) {
-                const PtRec<R> p = cd.pr;
-                const C cc = cd.cv;
-                const bool hv = cd.have;
-                cd.fetch(J, S, pb, v0 + NT + lane, t, total);
+                const PtRec<R> p = c0.pr;
+                const C cc = c0.cv;
+                const bool hv = c0.have;
+                c0 = c1;
+                c1.fetch(J, S, pb, v0 + 2 * NT + lane, t, total, e);
                 if (!hv) continue;
                 int fy = 0, ny = 1, my = 0, fz = 0, nz = 1, mz = 0;
                 if (g.dim >= 2) overlap(p.ly, y0, by, w, g.n2, fy, ny, my);
@@ -561,26 +564,22 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
             jn = job_of(tile0s, njobs, nxt);
             tile_decode(jobs[jn], nxt, tn, y0n, z0n);
             total_n = S.rpre[pb ^ 1][nruns_of(jobs[jn].g, y0n, z0n)];
-            cd.e = 0;
-            cd.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n);
+            e = 0;
+            c0.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n, e);
+            c1.fetch(jobs[jn], S, pb ^ 1, vw + NT + lane, tn, total_n, e);
         }
 
-        // ---- x FFT, one group of G rows at a time
+        // ---- x FFT of all rows at once (2E elements per thread): the first
+        // stage reads every group's accumulators before any row is written
         if (ok) {
-            auto group = [&](const int gi) __attribute__((always_inline)) {
-                C *gt = tile + gi * G * RSX;
-                const int *ga = acc + gi * GS32;
-                auto load_acc = [&](int row, int i) -> C {
-                    const int a = row * RS + i;
-                    return cmk<C>(ldexpf((float)ga[a], -q), ldexpf((float)ga[a + G * RS], -q));
-                };
-                auto load_tile = [&](int row, int i) -> C { return gt[row * RSX + padx<true>(i)]; };
-                auto store_tile = [&](int row, int i, C v) { gt[row * RSX + padx<true>(i)] = v; };
-                auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, gi * G + row, f, v); };
-                x_fft<C, E, NT>(J.rp1, n1, G, tw, load_acc, load_tile, store_tile, store_out);
+            auto load_acc = [&](int row, int i) -> C {
+                const int a = (row >> gsh) * GS32 + (row & (G - 1)) * RS + i;
+                return cmk<C>(ldexpf((float)acc[a], -q), ldexpf((float)acc[a + G * RS], -q));
             };
-            group(0);
-            if (rows > G) group(1);
+            auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
+            auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
+            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, row, f, v); };
+            x_fft<C, 2 * E, NT>(J.rp1, n1, rows, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
         __syncthreads();  // every read of the tile (and the twiddles) done
@@ -645,10 +644,9 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
     if (wv == 0) runs_publish(S, pb, jobs[j], t, y0, z0, lane, false);
     __syncthreads();
     Cand<R> cd;
-    cd.have = false;
-    cd.e = 0;
+    int e = 0;
     int total = S.rpre[pb][nruns_of(jobs[j].g, y0, z0)];
-    cd.fetch(jobs[j], S, pb, vw + lane, t, total);
+    cd.fetch(jobs[j], S, pb, vw + lane, t, total, e);
 
     for (;;) {
         const SpreadJob<R> &J = jobs[j];
@@ -699,7 +697,7 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
             const PtRec<R> p = cd.pr;
             const C cc = cd.cv;
             const bool hv = cd.have;
-            cd.fetch(J, S, pb, v0 + NT + lane, t, total);
+            cd.fetch(J, S, pb, v0 + NT + lane, t, total, e);
             if (!hv) continue;
             int fy = 0, ny = 1, my = 0, fz = 0, nz = 1, mz = 0;
             if (g.dim >= 2) overlap(p.ly, y0, by, w, g.n2, fy, ny, my);
@@ -770,8 +768,8 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
             jn = job_of(tile0s, njobs, nid);
             tile_decode(jobs[jn], nid, tn, y0n, z0n);
             total_n = S.rpre[pb ^ 1][nruns_of(jobs[jn].g, y0n, z0n)];
-            cd.e = 0;
-            cd.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n);
+            e = 0;
+            cd.fetch(jobs[jn], S, pb ^ 1, vw + lane, tn, total_n, e);
         }
 
         // ---- x FFT of the item's G rows
