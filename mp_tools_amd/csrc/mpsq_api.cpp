@@ -473,7 +473,8 @@ template <typename R> struct Core {
         g.by = by;
         g.bz = bz;
         // sort bins: start at the tile size, coarsen until the bin count fits
-        // the LDS counters and a tile visits at most MAX_NB_BINS bins
+        // the LDS counters, a tile visits at most MAX_NB_BINS bins and has at
+        // most spread_max_runs() runs
         int BY = (dim >= 2) ? by : 1, BZ = (dim >= 3) ? bz : 1;
         auto nvis = [&](int B, int b, int64_t nb) -> int64_t {
             int64_t v = (b + w - 2 + B - 1) / B + 1;

@@ -81,6 +81,14 @@ __device__ __forceinline__ double es_kernel(double t, double beta, double ic)
     return (a > 0.0) ? exp(beta * (sqrt(a) - 1.0)) : 0.0;
 }
 
+// nearest integer of x, ties up (floor(x + 0.5)): one v_cvt_rpi_i32_f32
+__device__ __forceinline__ int rint_up(float x)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // Rows [f, f + cnt) of a tile of b rows starting at grid row r0 covered by a
 // stencil of w cells starting at grid row l (periodic, period n; b + w <= n
 // so the covered rows are one contiguous range); m0 = first stencil index.
@@ -386,10 +394,10 @@ template <typename R> struct Cand {
     }
 };
 
-// kept frequency f of tile row r (deconvolved, conjugated if asked) to G1
+// kept frequency f of tile row r, times 2^e2 / phi_hat (conjugated if asked), to G1
 template <typename R>
 __device__ __forceinline__ void store_kept(const SpreadJob<R> &J, int t, int y0, int z0, int by, int r, int f,
-                                           typename cplx<R>::T v)
+                                           typename cplx<R>::T v, int e2)
 {
     const Geo &g = J.g;
     const int N1 = g.N1, h1 = N1 / 2, n1 = g.n1;
@@ -398,6 +406,7 @@ __device__ __forceinline__ void store_kept(const SpreadJob<R> &J, int t, int y0,
     else if (f >= n1 - h1) kk = f - (n1 - h1);
     else return;
     v = cscale(v, J.phinv1[kk]);
+    if (e2) v = cmk<typename cplx<R>::T>(ldexp(v.x, e2), ldexp(v.y, e2));
     if (J.final_conj) v.y = -v.y;
     const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
     J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
@@ -522,22 +531,27 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
 #pragma unroll
                 for (int iy = 0; iy < W; ++iy)
                     ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
+                R kz[W];
+#pragma unroll
+                for (int iz = 0; iz < W; ++iz)
+                    kz[iz] = (g.dim >= 3) ? ((iz < nz) ? es_kernel(p.dz + (R)(mz + iz), beta, ic) : (R)0) : (R)1;
+                const float csx = ldexpf(cc.x, q), csy = ldexpf(cc.y, q);  // strength * 2^q
                 for (int iz = 0; iz < nz; ++iz) {
-                    const R wz = (g.dim >= 3) ? es_kernel(p.dz + (R)(mz + iz), beta, ic) : (R)1;
+                    const R wz = kz[iz];  // iz is uniform over the lanes still in the loop
                     const int rr0 = (fz + iz) * by + fy;  // tile row of iy = 0
 #pragma unroll
                     for (int iy = 0; iy < W; ++iy) {
                         if (iy < ny) {
                             const int r = rr0 + iy;
                             const R wyz = ky[iy] * wz;
-                            const int c0 = (r >> gsh) * GS32 + (r & (G - 1)) * RS + p.lx;
-                            const float sr = ldexpf(cc.x * wyz, q), si = ldexpf(cc.y * wyz, q);
-                            int *ar = acc + c0, *ai = acc + c0 + G * RS;
+                            const int cell = (r >> gsh) * GS32 + (r & (G - 1)) * RS + p.lx;
+                            const float sr = csx * wyz, si = csy * wyz;
+                            int *ar = acc + cell, *ai = acc + cell + G * RS;
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
-                                __hip_atomic_fetch_add(ar + m, __float2int_rn(sr * kx[m]), __ATOMIC_RELAXED,
+                                __hip_atomic_fetch_add(ar + m, rint_up(sr * kx[m]), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_fetch_add(ai + m, __float2int_rn(si * kx[m]), __ATOMIC_RELAXED,
+                                __hip_atomic_fetch_add(ai + m, rint_up(si * kx[m]), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
@@ -574,11 +588,11 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
         if (ok) {
             auto load_acc = [&](int row, int i) -> C {
                 const int a = (row >> gsh) * GS32 + (row & (G - 1)) * RS + i;
-                return cmk<C>(ldexpf((float)acc[a], -q), ldexpf((float)acc[a + G * RS], -q));
+                return cmk<C>((float)acc[a], (float)acc[a + G * RS]);  // 2^-q goes into the output factor
             };
             auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
             auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
-            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, row, f, v); };
+            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, row, f, v, -q); };
             x_fft<C, 2 * E, NT>(J.rp1, n1, rows, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
@@ -786,7 +800,7 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
             };
             auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
             auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
-            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, r0 + row, f, v); };
+            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, r0 + row, f, v, 0); };
             x_fft<C, E, NT>(J.rp1, n1, G, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
