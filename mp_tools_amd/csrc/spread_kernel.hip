@@ -15,8 +15,8 @@
 //      with LDS atomics into planar accumulator rows (real plane, imaginary
 //      plane) that carry w ghost cells past x = n1, so a stencil never wraps
 //      inside the loop and every cell address is base + constant;
-//   3. folds the ghost cells back onto x < w, runs the x FFT of its rows in
-//      LDS (the first Stockham stage reads the accumulators, rows padded by
+//   3. runs the x FFT of its rows in LDS (the first Stockham stage reads the
+//      accumulators and folds the ghost cells back onto x < w, rows padded by
 //      one element per 16 against bank conflicts, twiddles in LDS), keeps the
 //      N1 central frequencies, divides by phi_hat and writes them: the only
 //      HBM write of the pass.
@@ -560,16 +560,6 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
             }
         }
         __syncthreads();
-
-        // ---- fold the ghost cells (32-bit adds wrap; the final sums are exact)
-        if (ok) {
-            for (int i = threadIdx.x; i < 2 * rows * W; i += NT) {
-                const int pl = i / (rows * W), rr = (i / W) % rows, x = i % W;
-                const int a = (rr >> gsh) * GS32 + pl * G * RS + (rr & (G - 1)) * RS;
-                acc[a + x] += acc[a + n1 + x];
-            }
-        }
-        __syncthreads();
         // the next tile's first candidates are in flight during this FFT
         const int nxt = S.next;
         const bool has_next = nxt < xend;
@@ -586,9 +576,16 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
         // ---- x FFT of all rows at once (2E elements per thread): the first
         // stage reads every group's accumulators before any row is written
         if (ok) {
+            // the ghost cells x = n1 + i fold onto i < w here (32-bit adds
+            // wrap; the final sums are exact); 2^-q goes into the output factor
             auto load_acc = [&](int row, int i) -> C {
                 const int a = (row >> gsh) * GS32 + (row & (G - 1)) * RS + i;
-                return cmk<C>((float)acc[a], (float)acc[a + G * RS]);  // 2^-q goes into the output factor
+                int re = acc[a], im = acc[a + G * RS];
+                if (i < W) {
+                    re += acc[a + n1];
+                    im += acc[a + G * RS + n1];
+                }
+                return cmk<C>((float)re, (float)im);
             };
             auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
             auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
@@ -765,15 +762,6 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
             }
         }
         __syncthreads();
-
-        // ---- fold the ghost cells
-        for (int i = threadIdx.x; i < 2 * G * W; i += NT) {
-            const int pl = i / (G * W), rr = (i / W) % G, x = i % W;
-            const int a = pl * wplane + rr * RS;
-            if (F32) acc64[a + x] += acc64[a + n1 + x];
-            else accd[a + x] += accd[a + n1 + x];
-        }
-        __syncthreads();
         const int kn = S.next;
         const bool has_next = kn < xend;
         int nid = 0, npass = 0, jn = j, tn = 0, y0n = 0, z0n = 0, total_n = 0;
@@ -788,14 +776,25 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
 
         // ---- x FFT of the item's G rows
         {
+            // the ghost cells x = n1 + i fold onto i < w here
             auto load_acc = [&](int row, int i) -> C {
                 const int a = row * RS + i;
                 if (F32) {
-                    const long long br = (long long)(acc64[a] << 16) >> 16;
-                    const long long bi = (long long)(acc64[wplane + a] << 16) >> 16;
+                    unsigned long long ur = acc64[a], ui = acc64[wplane + a];
+                    if (i < W) {
+                        ur += acc64[a + n1];
+                        ui += acc64[wplane + a + n1];
+                    }
+                    const long long br = (long long)(ur << 16) >> 16;
+                    const long long bi = (long long)(ui << 16) >> 16;
                     return cmk<C>(fbad ? (R)NAN : (R)((double)br * finv), fbad ? (R)NAN : (R)((double)bi * finv));
                 } else {
-                    return cmk<C>((R)accd[a], (R)accd[wplane + a]);
+                    double dr = accd[a], di = accd[wplane + a];
+                    if (i < W) {
+                        dr += accd[a + n1];
+                        di += accd[wplane + a + n1];
+                    }
+                    return cmk<C>((R)dr, (R)di);
                 }
             };
             auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
