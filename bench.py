@@ -165,8 +165,11 @@ def main():
     s_c = 8
     alg = {
         "spread_xfft": T * (M * (12 + s_c) + 3 * s_c * n_fine[2] * n_fine[1] * N[0]),
-        "colfft": T * 3 * s_c * (n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
-        "final_colfft": T * s_c * (3 * n_fine[2] * N[1] * N[0] + NQ),
+        # y pass: reads the 3 species' x-transformed planes, writes their
+        # weighted sum (scalar weights: summed before the y FFT); z pass:
+        # reads that sum, writes the modes
+        "colfft": T * s_c * (3 * n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
+        "final_colfft": T * s_c * (n_fine[2] * N[1] * N[0] + NQ),
         "reduce": T * NQ * s_c + NQ * (n_f + 1) * 8,
         "sort": T * M * (4 + 2 * 32 + 2 * s_c),
         "stage": T * M * (16 + 32 + 4 + s_c),

@@ -543,7 +543,7 @@ template <typename R> struct Core {
         }
         c.cols = colfft_cols(c.n, c.inner, final);
         c.dcols = make_fdiv((uint32_t)c.cols);
-        c.nsp = final ? nsp : 1;
+        c.nsp = nsp;
         c.NQ = NQ;
         c.conj = (final && iflag > 0) ? 1 : 0;
         return c;
@@ -558,7 +558,7 @@ template <typename R> struct Core {
         J.out = out;
         J.tw = tw[axis].template as<Cx<R>>();
         J.phinv = phinv[axis].template as<R>();
-        J.wsc = final ? wsc : nullptr;
+        J.wsc = wsc;
         J.wtab = final ? wtab : nullptr;
         J.ncb = (J.cp.inner + J.cp.cols - 1) / J.cp.cols;
         J.ngroups = ngroups;
@@ -1271,13 +1271,20 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                             : parts[q].wtab ? parts[q].wtab
                             : (e.wtab.p ? e.wtab.template as<float>() : nullptr);
         if (c.dim == 3) {
-            ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
+            // scalar species weights: the weighted species sum is taken before
+            // the y FFT (linear), so the y pass writes one plane per frame and
+            // the z pass reads a third of the data; per-Q weights (X-ray form
+            // factors) are applied per species after the z FFT
+            const bool early = e.combine && !wtab;
+            ColJob<R> y = early ? c.coljob(1, false, nspg, ngrp, g1, g2, wsc, nullptr)
+                                : c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
             y.blk0 = cyn;
             cyn += (int)Core<R>::col_blocks(y);
             cyl = std::max(cyl, colfft_lds<R>(y.cp, false));
             cyj.push_back(y);
             cyb.push_back(y.blk0);
-            ColJob<R> z = c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
+            ColJob<R> z = early ? c.coljob(2, true, 1, ngrp, g2, out, nullptr, nullptr)
+                                : c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
             z.blk0 = czn;
             czn += (int)Core<R>::col_blocks(z);
             czl = std::max(czl, colfft_lds<R>(z.cp, true));

@@ -454,14 +454,22 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
         const int t = FINAL ? grp * cp.nsp + s : grp;
         {
             // all E loads of a thread are issued before the first LDS store;
-            // tile element i = threadIdx.x + k * NT is row i / cols, column i % cols
-            const C *src = J.in + (int64_t)t * cp.in_tstride + (int64_t)o * n * inner + col0;
+            // tile element i = threadIdx.x + k * NT is row i / cols, column i % cols.
+            // A non-final pass with nsp > 1 sums the group's nsp transforms
+            // with the weights wsc before its FFT (the FFT is linear).
+            const int nspl = FINAL ? 1 : cp.nsp;
             C v[E];
+            for (int u = 0; u < nspl; ++u) {
+                const int tu = FINAL ? t : grp * nspl + u;
+                const C *src = J.in + (int64_t)tu * cp.in_tstride + (int64_t)o * n * inner + col0;
+                const R wu = (!FINAL && J.wsc) ? J.wsc[u] : (R)1;
 #pragma unroll
-            for (int k = 0; k < E; ++k) {
-                const int i = threadIdx.x + k * NT;
-                const int r = (int)fdiv((uint32_t)i, cp.dcols), c = i - r * cols;
-                v[k] = (i < n * cols && c < ncol) ? src[r * inner + c] : cmk<C>((R)0, (R)0);
+                for (int k = 0; k < E; ++k) {
+                    const int i = threadIdx.x + k * NT;
+                    const int r = (int)fdiv((uint32_t)i, cp.dcols), c = i - r * cols;
+                    const C x = (i < n * cols && c < ncol) ? src[r * inner + c] : cmk<C>((R)0, (R)0);
+                    v[k] = (u == 0) ? cscale(x, wu) : cadd(v[k], cscale(x, wu));
+                }
             }
 #pragma unroll
             for (int k = 0; k < E; ++k) {
