@@ -163,12 +163,11 @@ def main():
     # roofline of the dominant kernel (per-kernel algorithmic bytes)
     n_fine = (2 * N[0], 2 * N[1], 2 * N[2])
     s_c = 8
+    # scalar species weights: a frame is one transform over all atoms with
+    # strengths w_s c_j, so every pass after the spread moves one plane per frame
     alg = {
-        "spread_xfft": T * (M * (12 + s_c) + 3 * s_c * n_fine[2] * n_fine[1] * N[0]),
-        # y pass: reads the 3 species' x-transformed planes, writes their
-        # weighted sum (scalar weights: summed before the y FFT); z pass:
-        # reads that sum, writes the modes
-        "colfft": T * s_c * (3 * n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
+        "spread_xfft": T * (M * (12 + s_c) + s_c * n_fine[2] * n_fine[1] * N[0]),
+        "colfft": T * s_c * (n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
         "final_colfft": T * s_c * (n_fine[2] * N[1] * N[0] + NQ),
         "reduce": T * NQ * s_c + NQ * (n_f + 1) * 8,
         "sort": T * M * (4 + 2 * 32 + 2 * s_c),

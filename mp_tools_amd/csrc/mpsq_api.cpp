@@ -599,6 +599,7 @@ template <typename R> struct Core {
         J.phinv1 = phinv[0].template as<R>();
         J.G1 = G1;
         J.amax = amax;
+        J.amax_s = 1.0f;
         J.binmax = bmax;
         J.final_conj = final_conj;
         J.ntr = ntr;
@@ -909,6 +910,7 @@ template <typename R> struct Part {
     const float *pos;  // frame f0 of the trajectory [T][n_tot][4]
     void *amp;         // output of frame f0 ([b][NQ] combined, [b][nact][NQ] per species)
     const R *wsc = nullptr;      // species weights for this part (nullptr: the engine's)
+    const double *wsh = nullptr; // host copy of wsc (nullptr: the engine's)
     const float *wtab = nullptr; // per-Q weight table [nact][NQ] (nullptr: the engine's, if any)
 };
 
@@ -928,6 +930,7 @@ template <typename R> struct Engine {
     StageParams sp{};
     int B = 1;                      // frames per batch when run alone
     DevBuf wsc, wtab, at_ind;
+    std::vector<double> wsh;        // active species weights (host copy of wsc)
     Workspace<R> ws;                // buffers when run alone
     // reduction caches
     DevBuf win_d, twT;
@@ -940,10 +943,12 @@ template <typename R> struct Engine {
     int64_t nq() const { return core.N[0] * core.N[1] * core.N[2]; }
     int64_t amp_frame_elems() const { return combine ? nq() : nq() * nact; }
     // device bytes of intermediates per frame (points, sort keys, grids)
-    size_t bytes_per_frame() const
+    // (a frame is one transform when its species are merged, else nact; the
+    // default counts nact, which bounds both)
+    size_t bytes_per_frame(int transforms = 0) const
     {
         return (size_t)Mact * (2 * sizeof(PtRec<R>) + 2 * sizeof(Cx<R>) + 2 * sizeof(int)) +
-               (size_t)nact * sizeof(Cx<R>) *
+               (size_t)(transforms > 0 ? transforms : nact) * sizeof(Cx<R>) *
                    (size_t)(core.n[2] * core.n[1] * core.N[0] +
                             (core.dim == 3 ? core.n[2] * core.N[1] * core.N[0] : 0));
     }
@@ -979,6 +984,7 @@ template <typename R> struct Engine {
         // weights of the active species
         std::vector<R> wh((size_t)nact);
         for (int k = 0; k < nact; ++k) wh[k] = s->w_scalar ? (R)s->w_scalar[act[k]] : (R)1;
+        wsh.assign(wh.begin(), wh.end());
         wsc.ensure(sizeof(R) * (size_t)nact);
         HIPCHK(hipMemcpy(wsc.p, wh.data(), sizeof(R) * (size_t)nact, hipMemcpyHostToDevice));
         if (s->w_table) {
@@ -1009,8 +1015,9 @@ template <typename R> struct Engine {
             HIPCHK(hipMemcpy(at_ind.p, s->at_ind, sizeof(int) * 4 * (size_t)n_tot, hipMemcpyHostToDevice));
         }
         // batch size: intermediate buffers of about 2 GiB per batch, <= 16 frames
+        const int tpf = (combine && !wtab.p) ? 1 : nact;  // transforms per frame when run alone
         int b = s->batch_frames > 0 ? s->batch_frames
-                                    : (int)std::max<size_t>(1, ((size_t)2 << 30) / bytes_per_frame());
+                                    : (int)std::max<size_t>(1, ((size_t)2 << 30) / bytes_per_frame(tpf));
         B = std::max(1, std::min(b, 16));
     }
 
@@ -1128,10 +1135,25 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     std::vector<TransformInfo> tinfo;
     int max_nbins = 1;
     int sblocks = 0;
+    // A combined map with scalar species weights is one transform per frame
+    // over all active atoms with strengths w_s c_j (the NUFFT is linear in
+    // the strengths); per-species amplitudes or per-Q weight tables keep one
+    // transform per species.
+    std::vector<char> merged(np);
+    std::vector<float> wbound(np, 1.0f);
+    auto part_wtab = [&](int q) -> const float * {
+        const Engine<R> &e = *parts[q].e;
+        return !e.combine ? nullptr : parts[q].wtab ? parts[q].wtab : (e.wtab.p ? e.wtab.template as<float>() : nullptr);
+    };
+    for (int q = 0; q < np; ++q) {
+        const Engine<R> &e = *parts[q].e;
+        merged[q] = e.combine && !part_wtab(q);
+    }
+    auto tpf = [&](int q) { return merged[q] ? 1 : parts[q].e->nact; };  // transforms per frame
     for (int q = 0; q < np; ++q) {
         const Engine<R> &e = *parts[q].e;
         const Core<R> &c = e.core;
-        const int ntr = parts[q].b * e.nact;
+        const int ntr = parts[q].b * tpf(q);
         p0[q + 1] = p0[q] + (int64_t)parts[q].b * e.Mact;
         t0[q + 1] = t0[q] + ntr;
         g1o[q + 1] = g1o[q] + (int64_t)ntr * c.n[2] * c.n[1] * c.N[0];
@@ -1150,19 +1172,27 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         J.mact = e.Mact;
         J.blk0 = sblocks;
         J.nact = e.nact;
+        const double *wq = parts[q].wsh ? parts[q].wsh : e.wsh.data();
+        double wmax = 0.0;
         for (int a = 0; a < e.nact; ++a) {
             J.act_src[a] = e.sp_off[e.act[a]];
             J.act_off[a] = e.act_off[a];
+            J.wst[a] = merged[q] ? wq[a] : 1.0;
+            wmax = std::max(wmax, std::fabs(wq[a]));
         }
         J.act_off[e.nact] = e.act_off[e.nact];
+        // bound of the staged strengths' components (|w_s c_j| <= |w_s|),
+        // rounded up past the staging's FP32 rounding
+        if (merged[q]) wbound[q] = (float)(wmax * (1.0 + 1.0 / (1 << 20)));
         stb[q] = sblocks;
         sblocks += (int)((J.npts + 255) / 256);
-        // sort chunks of every transform (frame fb, active species a)
+        // sort chunks of every transform (frame fb, active species a, or
+        // frame fb when merged)
         const int64_t CH = c.sort_chunk;
         for (int fb = 0; fb < parts[q].b; ++fb)
-            for (int a = 0; a < e.nact; ++a) {
-                const int64_t pa = p0[q] + (int64_t)fb * e.Mact + e.act_off[a];
-                const int64_t pb = p0[q] + (int64_t)fb * e.Mact + e.act_off[a + 1];
+            for (int a = 0; a < tpf(q); ++a) {
+                const int64_t pa = p0[q] + (int64_t)fb * e.Mact + (merged[q] ? 0 : e.act_off[a]);
+                const int64_t pb = p0[q] + (int64_t)fb * e.Mact + (merged[q] ? e.Mact : e.act_off[a + 1]);
                 const int nch = (int)((pb - pa + CH - 1) / CH);
                 for (int k = 0; k < nch; ++k) {
                     SortChunk ch;
@@ -1180,7 +1210,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                 ti.p0 = pa;
                 ti.p1 = pb;
                 ti.hbase = hsz;
-                ti.bs0 = bso[q] + (int64_t)(fb * e.nact + a) * (c.g.nbins + 1);
+                ti.bs0 = bso[q] + (int64_t)(fb * tpf(q) + a) * (c.g.nbins + 1);
                 ti.nch = nch;
                 ti.nbins = c.g.nbins;
                 c.segments(ti.sxsh, ti.nseg);
@@ -1235,16 +1265,17 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             const Engine<R> &e = *parts[q].e;
             const Core<R> &c = e.core;
             if (sclass(c) != widths[wi]) continue;
-            const int ntr = parts[q].b * e.nact;
+            const int ntr = parts[q].b * tpf(q);
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), ws.cs.template as<Cx<R>>(), 0,
                                          ws.binstart.template as<int64_t>() + bso[q],
                                          sizeof(R) == 4 ? ws.binmax.template as<int>() + bso[q] : nullptr, 1,
                                          ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
+            J.amax_s = wbound[q];
             J.tile0 = tile;
             tile += J.ntiles * ntr;
             sjobs[wi].push_back(J);
             st0[wi].push_back(J.tile0);
-            cost.push_back(c.tile_cost(e.Mact / std::max(1, e.nact)));
+            cost.push_back(c.tile_cost(e.Mact / tpf(q)));
             ntl.push_back(J.ntiles * ntr);
             lds = std::max(lds, c.spread_lds_size());
         }
@@ -1260,31 +1291,24 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     for (int q = 0; q < np; ++q) {
         const Engine<R> &e = *parts[q].e;
         const Core<R> &c = e.core;
-        const int ntr = parts[q].b * e.nact;
+        const int ntr = parts[q].b * tpf(q);
         const int ngrp = e.combine ? parts[q].b : ntr;
-        const int nspg = e.combine ? e.nact : 1;
+        const int nspg = merged[q] ? 1 : e.combine ? e.nact : 1;
         Cx<R> *g1 = ws.G1.template as<Cx<R>>() + g1o[q];
         Cx<R> *g2 = ws.G2.template as<Cx<R>>() + g2o[q];
         Cx<R> *out = static_cast<Cx<R> *>(parts[q].amp);
-        const R *wsc = e.combine ? (parts[q].wsc ? parts[q].wsc : e.wsc.template as<R>()) : nullptr;
-        const float *wtab = !e.combine ? nullptr
-                            : parts[q].wtab ? parts[q].wtab
-                            : (e.wtab.p ? e.wtab.template as<float>() : nullptr);
+        // merged: weights are in the strengths; per-Q weights (X-ray form
+        // factors) are applied per species after the last FFT
+        const R *wsc = (e.combine && !merged[q]) ? (parts[q].wsc ? parts[q].wsc : e.wsc.template as<R>()) : nullptr;
+        const float *wtab = part_wtab(q);
         if (c.dim == 3) {
-            // scalar species weights: the weighted species sum is taken before
-            // the y FFT (linear), so the y pass writes one plane per frame and
-            // the z pass reads a third of the data; per-Q weights (X-ray form
-            // factors) are applied per species after the z FFT
-            const bool early = e.combine && !wtab;
-            ColJob<R> y = early ? c.coljob(1, false, nspg, ngrp, g1, g2, wsc, nullptr)
-                                : c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
+            ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
             y.blk0 = cyn;
             cyn += (int)Core<R>::col_blocks(y);
             cyl = std::max(cyl, colfft_lds<R>(y.cp, false));
             cyj.push_back(y);
             cyb.push_back(y.blk0);
-            ColJob<R> z = early ? c.coljob(2, true, 1, ngrp, g2, out, nullptr, nullptr)
-                                : c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
+            ColJob<R> z = c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
             z.blk0 = czn;
             czn += (int)Core<R>::col_blocks(z);
             czl = std::max(czl, colfft_lds<R>(z.cp, true));
@@ -1584,6 +1608,7 @@ struct SrvReq {
     void *amp = nullptr;
     double *S = nullptr, *St = nullptr, *coef = nullptr;
     float *wsc = nullptr, *wtab = nullptr;
+    std::vector<double> wact;  // host copy of wsc
     int64_t next = 0;  // frames already given to a step
     bool reducing = false;
     hipEvent_t done_ev = nullptr;
@@ -1705,6 +1730,7 @@ struct Server {
         for (int j = 0; j < tj.nsp; ++j) {
             if (!q->mask.empty() && !q->mask[j]) continue;
             wa.push_back(q->r.weight_kind == 0 ? (float)q->w[j] : 1.0f);
+            q->wact.push_back((double)wa.back());
             if (q->r.weight_kind == 1) ca.insert(ca.end(), q->xff.begin() + 9 * j, q->xff.begin() + 9 * j + 9);
         }
         HIPCHK(hipMallocAsync(reinterpret_cast<void **>(&q->wsc), sizeof(float) * wa.size(), st));
@@ -1790,6 +1816,7 @@ struct Server {
                 p.pos = tj.pos + (size_t)p.f0 * tj.n_tot * 4;
                 p.amp = static_cast<float2 *>(q->amp) + (size_t)q->next * q->nq;
                 p.wsc = q->wsc;
+                p.wsh = q->wact.data();
                 p.wtab = q->wtab;
                 parts.push_back(p);
                 in_step.push_back(q);

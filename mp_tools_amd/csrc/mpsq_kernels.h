@@ -99,6 +99,8 @@ struct StageJob {
     int nact;
     int64_t act_src[64]; // first atom of active species k in a frame
     int64_t act_off[65]; // prefix of active species counts
+    double wst[64];      // strength factor of active species k (its weight when the
+                         // species of a frame form one transform, else 1)
 };
 
 // spread + x FFT of ntr transforms sharing one geometry; tiles
@@ -114,7 +116,8 @@ template <typename R> struct SpreadJob {
     const typename cplx<R>::T *tw1;
     const R *phinv1;
     typename cplx<R>::T *G1;             // [ntr][n3][n2][N1]
-    const float *amax;                   // per transform, or nullptr (strengths bounded by 1)
+    const float *amax;                   // per transform, or nullptr (strengths bounded by amax_s)
+    float amax_s;
     const int *binmax;                   // FP32: per-bin covering bound, indexed like binstart (or nullptr)
     int grp, grp_sh;                     // rows per x-FFT group (a power of two) and its log2
     int final_conj;

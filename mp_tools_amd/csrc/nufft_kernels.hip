@@ -134,6 +134,7 @@ __global__ __launch_bounds__(256) void k_stage_traj(const StageJob *__restrict__
     int k = 0;
     while (k + 1 < J.nact && J.act_off[k + 1] <= aa) ++k;
     const int64_t a = J.act_src[k] + (aa - J.act_off[k]);
+    const double wk = J.wst[k];
     const float4 r = J.pos[fb * J.nspan + a];
     const int4 *at_ind = J.at_ind;
     PtRec<R> *rec = rec_all + J.out0;
@@ -189,6 +190,8 @@ __global__ __launch_bounds__(256) void k_stage_traj(const StageJob *__restrict__
             ci *= wf;
         }
         if (sp.conj) ci = -ci;
+        cr *= wk;
+        ci *= wk;
     }
     PtRec<R> rr;
     rr.lx = l[0];

@@ -369,7 +369,7 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
     }
     if (lane == 0) {
         S.rpre[pb][0] = 0;
-        S.amax[pb] = J.amax ? J.amax[t] : 1.0f;
+        S.amax[pb] = J.amax ? J.amax[t] : J.amax_s;
         S.cb[pb] = cb;
     }
 }
