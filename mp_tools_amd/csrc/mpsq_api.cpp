@@ -609,6 +609,8 @@ template <typename R> struct Core {
         J.grp = spread_group_rows<R>(by * bz, g.n1);
         J.grp_sh = ilog2_ceil(J.grp);
         J.dntr = make_fdiv((uint32_t)ntr);
+        J.dnt = make_fdiv((uint32_t)J.ntiles);
+        J.tmajor = 0;
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
         J.lds_tw = (int)(spread_lds_size() - (size_t)g.n1 * sizeof(Cx<R>));
@@ -1271,6 +1273,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                                          sizeof(R) == 4 ? ws.binmax.template as<int>() + bso[q] : nullptr, 1,
                                          ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
             J.amax_s = wbound[q];
+            J.tmajor = merged[q] ? 1 : 0;
             J.tile0 = tile;
             tile += J.ntiles * ntr;
             sjobs[wi].push_back(J);

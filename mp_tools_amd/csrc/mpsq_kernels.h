@@ -122,7 +122,8 @@ template <typename R> struct SpreadJob {
     int grp, grp_sh;                     // rows per x-FFT group (a power of two) and its log2
     int final_conj;
     int ntr, ntiles, tile0;
-    FDiv dntr, dnty, dby;
+    int tmajor;                          // tile ids transform-major (else position-major)
+    FDiv dntr, dnty, dby, dnt;           // ntr, tiles along y, by, ntiles
     int lds_tw;                          // byte offset of the twiddles in dynamic LDS
 };
 

@@ -241,16 +241,24 @@ __device__ __forceinline__ int job_of(const int *__restrict__ tile0s, int njobs,
     return __builtin_amdgcn_readfirstlane(lo);  // uniform: id is the workgroup's tile
 }
 
-// Tile ids: job-major; inside a job, id - tile0 = (tile position) * ntr +
-// transform, so the tiles taken one after another cover every transform of
-// a few neighbouring positions (their candidates overlap and come from the
-// XCD's L2), and the species (3x different point counts) are spread evenly.
+// Tile ids: job-major.  Inside a job whose transforms differ in cost (one
+// per species), id - tile0 = (tile position) * ntr + transform, so the XCDs'
+// ranges get the same mix of species (3x different point counts); when they
+// cost the same (one per frame, tmajor), id - tile0 = transform * ntiles +
+// position, so the tiles an XCD runs at once are neighbours in one grid and
+// share their candidates in its L2.
 template <typename R>
 __device__ __forceinline__ void tile_decode(const SpreadJob<R> &J, int id, int &t, int &y0, int &z0)
 {
     const int loc = id - J.tile0;
-    const int ti = (int)fdiv((uint32_t)loc, J.dntr);
-    t = loc - ti * J.ntr;
+    int ti;
+    if (J.tmajor) {
+        t = (int)fdiv((uint32_t)loc, J.dnt);
+        ti = loc - t * J.ntiles;
+    } else {
+        ti = (int)fdiv((uint32_t)loc, J.dntr);
+        t = loc - ti * J.ntr;
+    }
     const int by = (J.g.dim >= 2) ? J.g.by : 1, bz = (J.g.dim >= 3) ? J.g.bz : 1;
     const int tz = (int)fdiv((uint32_t)ti, J.dnty);
     y0 = (ti - tz * ((J.g.dim >= 2) ? J.g.n2 / by : 1)) * by;
