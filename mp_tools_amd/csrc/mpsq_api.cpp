@@ -782,7 +782,7 @@ template <typename R> struct Plan {
         const size_t o_cy = run_tab.put(cy), o_cz = run_tab.put(cz);
         run_tab.upload(c.st);
         wc.ensure(sizeof(int) * XCD_CTR_INTS);
-        fbl.ensure(sizeof(int) * 2 * (size_t)ntile);
+        fbl.ensure(sizeof(int) * (size_t)spread_max_wide_passes() * (size_t)ntile);
         launch_spread_jobs<R>(run_tab.dev<SpreadJob<R>>(o_sj), run_tab.dev<int>(o_z), 1, c.g.w,
                               run_tab.dev<int>(o_xr), wc.as<int>(), fbl.as<int>(), c.spread_lds_size(), c.st);
         if (!cy.empty())
@@ -1286,7 +1286,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size()});
         max_tiles = std::max(max_tiles, tile);
     }
-    ws.fbl.ensure(sizeof(int) * 2 * (size_t)std::max(max_tiles, 1));
+    ws.fbl.ensure(sizeof(int) * (size_t)spread_max_wide_passes() * (size_t)std::max(max_tiles, 1));
     std::vector<ColJob<R>> cyj, czj;
     std::vector<int> cyb, czb;
     int cyn = 0, czn = 0;

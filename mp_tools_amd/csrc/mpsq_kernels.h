@@ -174,6 +174,8 @@ void launch_gather(int64_t M, int ntr, const int *perm, const typename cplx<R>::
 template <typename R> int spread_tile_elems();
 int spread_max_runs();
 int spread_max_tile_b();
+// wide passes a queued FP32 tile can take (entries of fbl per tile)
+int spread_max_wide_passes();
 // largest tile (rows of the whole x extent) of a fine grid of n1 cells in x,
 // and the rows one x-FFT group of such a tile holds
 template <typename R> int spread_max_rows(int n1);
@@ -190,8 +192,9 @@ template <typename R> int colfft_tile_elems(bool final);
 // Workgroups run on XCD blockIdx.x % 8 and take tiles of their XCD's range
 // [xr[x], xr[x+1]) of the tile-id space from the counter
 // wc[x * XCD_CTR_STRIDE] (wc: XCD_CTR_INTS ints, zeroed by this call).  FP32:
-// tiles too dense for 32-bit accumulation are queued in fbl (2 ints per
-// tile of the table) and run by a second, 64-bit launch.  All jobs share the
+// tiles too dense for 32-bit accumulation are queued in fbl
+// (spread_max_wide_passes() ints per tile of the table) and run by a second,
+// 64-bit launch.  All jobs share the
 // kernel width w; lds = max over jobs of spread_lds.
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,

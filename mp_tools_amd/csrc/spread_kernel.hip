@@ -50,9 +50,19 @@
 
 namespace mpsq {
 
+// wide kernel: 512 threads, x FFT of up to E * NT cells (G rows) per pass
 template <typename R> struct SpreadCfg;
 template <> struct SpreadCfg<float> { static constexpr int E = 8, NT = 512; };
 template <> struct SpreadCfg<double> { static constexpr int E = 8, NT = 512; };
+// narrow kernel: 512 threads whose tile (up to 16 rows of a 512-cell grid,
+// ~74 KB of LDS, two workgroups per CU) is transformed at once, 2E cells per
+// thread.  (One 1024-thread workgroup per CU with 32-row tiles reads each
+// point 32% less often but ran 13% slower: with a single workgroup per CU
+// nothing overlaps its barrier-separated phases.)
+struct NarrowCfg {
+    static constexpr int E = 8, NT = 512;
+};
+constexpr int MAX_WIDE_PASSES = 4;  // narrow tile rows / wide pass rows
 
 constexpr int MAX_RUNS = 64;     // candidate runs of one tile (two per visited z-bin row)
 constexpr int MAX_TILE_B = 16;   // by, bz <= 16
@@ -126,10 +136,10 @@ __host__ __device__ inline int spread_row_stride_x(int n1, int w)
 }
 template <typename R> int spread_max_rows(int n1)
 {
-    const int te = SpreadCfg<R>::E * SpreadCfg<R>::NT;
+    const int te = (sizeof(R) == 4) ? 2 * NarrowCfg::E * NarrowCfg::NT : SpreadCfg<R>::E * SpreadCfg<R>::NT;
     int g = 1;
     while (2 * g * n1 <= te) g *= 2;
-    return sizeof(R) == 4 ? 2 * g : g;
+    return g;
 }
 template <typename R> int spread_group_rows(int rows, int n1)
 {
@@ -436,17 +446,18 @@ template <typename R, int NT> __device__ __forceinline__ void load_twiddles(cons
 // tile's id and loads its runs and bounds; while it is transformed, every
 // lane already loads its first candidate of the next tile.  A tile whose
 // covering bound exceeds NARROW_MAX (or whose strengths are not finite) is
-// appended to fbl as its wide passes (tile id * 2 + pass) instead.
+// appended to fbl as its wide passes (tile id * MAX_WIDE_PASSES + pass)
+// instead.
 // ---------------------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float> *__restrict__ jobs,
+__global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const SpreadJob<float> *__restrict__ jobs,
                                                           const int *__restrict__ tile0s, int njobs,
                                                           const int *__restrict__ xr, int *__restrict__ wc,
                                                           int *__restrict__ fbl)
 {
     using R = float;
     using C = float2;
-    constexpr int E = SpreadCfg<float>::E, NT = SpreadCfg<float>::NT;
+    constexpr int E = NarrowCfg::E, NT = NarrowCfg::NT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ RunTable S;
 
@@ -502,7 +513,7 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
         } else if (threadIdx.x == 0) {
             const int np = rows >> gsh;
             const int k = atomicAdd(wc + CTR_FB_COUNT, np);
-            for (int p = 0; p < np; ++p) fbl[k + p] = cur * 2 + p;
+            for (int p = 0; p < np; ++p) fbl[k + p] = cur * MAX_WIDE_PASSES + p;
         }
         __syncthreads();
         if (wv == 0) {
@@ -617,7 +628,8 @@ __global__ __launch_bounds__(512, 4) void k_spread_narrow(const SpreadJob<float>
 // Wide spread + x FFT: one pass of G rows of a tile per item.  Items are
 // either every tile of the table (fbl == nullptr; per-XCD ranges as in the
 // narrow kernel) or the list fbl[0 .. wc[CTR_FB_COUNT]) the narrow kernel
-// queued (item = tile id * 2 + pass), taken through wc[CTR_FB_TAKE].
+// queued (item = tile id * MAX_WIDE_PASSES + pass), taken through
+// wc[CTR_FB_TAKE].
 // ---------------------------------------------------------------------------
 template <typename R, int W>
 __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__restrict__ jobs,
@@ -643,10 +655,10 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
         return fbl ? atomicAdd(wc + CTR_FB_TAKE, 1) : xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
     };
     auto item_of = [&](int k, int &id, int &pass) {
-        int it = fbl ? fbl[k] : 2 * k;
+        int it = fbl ? fbl[k] : MAX_WIDE_PASSES * k;
         it = __builtin_amdgcn_readfirstlane(it);
-        id = it >> 1;
-        pass = it & 1;
+        id = it / MAX_WIDE_PASSES;
+        pass = it % MAX_WIDE_PASSES;
     };
 
     if (threadIdx.x == 0) S.next = take();
@@ -826,6 +838,7 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
 template <typename R> int spread_tile_elems() { return SpreadCfg<R>::E * SpreadCfg<R>::NT; }
 int spread_max_runs() { return MAX_RUNS; }
 int spread_max_tile_b() { return MAX_TILE_B; }
+int spread_max_wide_passes() { return MAX_WIDE_PASSES; }
 template <typename R> size_t spread_lds(int rows, int n1, int w) { return spread_lds_bytes<R>(rows, n1, w); }
 
 __global__ void k_zero_i32(int *__restrict__ p, int n)
@@ -869,8 +882,8 @@ static void launch_w(const SpreadJob<R> *jobs, const int *tile0s, int njobs, con
     const void *fw = reinterpret_cast<const void *>(&k_spread_wide<R, W>);
     if constexpr (sizeof(R) == 4) {
         const void *fn = reinterpret_cast<const void *>(&k_spread_narrow<W>);
-        hipLaunchKernelGGL((k_spread_narrow<W>), dim3((unsigned)persistent_grid(fn, NT, lds)), dim3(NT), lds, st, jobs,
-                           tile0s, njobs, xr, wc, fbl);
+        hipLaunchKernelGGL((k_spread_narrow<W>), dim3((unsigned)persistent_grid(fn, NarrowCfg::NT, lds)),
+                           dim3(NarrowCfg::NT), lds, st, jobs, tile0s, njobs, xr, wc, fbl);
         // the tiles the narrow kernel queued (usually none: its workgroups then
         // only read the empty count)
         hipLaunchKernelGGL((k_spread_wide<R, W>), dim3((unsigned)persistent_grid(fw, NT, lds)), dim3(NT), lds, st,
