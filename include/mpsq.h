@@ -279,6 +279,19 @@ int mpsq_server_wait_idle(mpsq_server srv);
 int mpsq_server_stats(mpsq_server srv, int64_t *steps, int64_t *done, double *busy_ms);
 int mpsq_server_destroy(mpsq_server srv);
 
+/* ---------------- mp_sql: spherically averaged partial intensities ----------------
+ * Replaces the Gauss integration loop of mp_sql (code/src/mp_sql56.f90:1224-1254;
+ * q_norm table :1036-1044): for every mode k of the nsp amplitude grids
+ * d_amp ([nsp][N3][N2][N1] complex, FP32 or FP64 by precision, CMCL order,
+ * k_d = i_d - N_d/2), q = |a_cell_inv . (k * dq)| in FP32 and, for q <= q_range,
+ * d_hist[nint(q / q_step)][p] += Re(A_i A_j*) for the pairs p = (i >= j) in the
+ * order (0,0), (1,0), (1,1), (2,0), ... (nsp (nsp + 1) / 2 of them).  d_hist is
+ * device double [n_hist][npairs], accumulated (not zeroed); a_cell_inv row-major
+ * (v_i = sum_j a_cell_inv[3 i + j] u_j).  Enqueued on stream (NULL: the default
+ * stream); n_hist * npairs * 8 bytes must fit 64 KB (else MPSQ_ERR_SIZE). */
+int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t n_modes[3], const float dq[3],
+                  const float a_cell_inv[9], float q_range, float q_step, int n_hist, double *d_hist, void *stream);
+
 /* Library version string. */
 const char *mpsq_version(void);
 

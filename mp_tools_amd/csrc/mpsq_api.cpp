@@ -1903,6 +1903,33 @@ void mpsq_default_opts(mpsq_opts *o)
 
 const char *mpsq_version(void) { return "mpsq 0.1 (gfx950)"; }
 
+int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
+                  const float *a_cell_inv, float q_range, float q_step, int n_hist, double *d_hist, void *stream)
+{
+    if (!d_amp || !n_modes || !dq || !a_cell_inv || !d_hist || nsp < 1 || nsp > 8 || n_hist < 1 ||
+        !(q_step > 0.0f) || (precision != MPSQ_FP32 && precision != MPSQ_FP64))
+        return MPSQ_ERR_BAD_ARG;
+    SqlGeom g{};
+    int64_t nm = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (n_modes[d] < 1 || n_modes[d] > (int64_t)1 << 20) return MPSQ_ERR_BAD_ARG;
+        g.N[d] = (int)n_modes[d];
+        g.dq[d] = dq[d];
+        nm *= n_modes[d];
+    }
+    for (int i = 0; i < 9; ++i) g.ainv[i] = a_cell_inv[i];
+    g.q_range = q_range;
+    g.q_step = q_step;
+    g.n_hist = n_hist;
+    g.nsp = nsp;
+    g.npairs = nsp * (nsp + 1) / 2;
+    g.nmodes = nm;
+    const int rc = launch_sql_hist(d_amp, precision == MPSQ_FP64, g, d_hist, static_cast<hipStream_t>(stream));
+    if (rc == -1) return MPSQ_ERR_SIZE;
+    if (rc != 0 || hipGetLastError() != hipSuccess) return MPSQ_ERR_DEVICE;
+    return 0;
+}
+
 // ---- FP64 plan ----
 int mpsq_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntrans, double eps, mpsq_plan *plan,
                   const mpsq_opts *opts)

@@ -199,6 +199,19 @@ template <typename R> int colfft_tile_elems(bool final);
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
                         int *fbl, size_t lds, hipStream_t st);
+// |Q| histogram of partial intensities (mp_sql's Gauss integration)
+struct SqlGeom {
+    int N[3];
+    float dq[3];
+    float ainv[9];  // row-major: v_i = sum_j ainv[3 i + j] u_j
+    float q_range, q_step;
+    int n_hist, nsp, npairs;
+    int64_t nmodes;
+};
+// out[n_hist][npairs] += the histogram of amp ([nsp][nmodes] complex);
+// returns 0, -1 if the histogram does not fit one workgroup's LDS (64 KB)
+// or nsp > 8, -2 if the partial buffer could not be allocated
+int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st);
 // p[0..n) = 0 by a kernel (graph-capturable, unlike a memset node)
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)

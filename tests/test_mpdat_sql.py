@@ -1,0 +1,75 @@
+"""MP_TOOLS .dat reader and the host side of the mp_sql path (CPU).
+
+The snapshot tests/golden/al_u/Al_U_large_n0001.dat is the reference's own
+example input (examples/amorphous_Al92U08/data); its header and species
+partition are known answers: 27,600 Al + 2,400 U, box 79.8159637 A, identity
+cell matrices, at_ind(1) = serial number, at_ind(4) = species
+(code/docs/1_mp_data_doc_154.txt:65-113).
+"""
+import os
+
+import numpy as np
+
+from mp_tools_amd import mpdat, sql
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DAT = os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat")
+
+
+def test_namelist_parser():
+    d = mpdat.parse_namelist('&H  A= 3*1.5 , N=2, S="ab c  ", V= 1.0, 2.5E-01, /')
+    assert d["A"] == [1.5, 1.5, 1.5] and d["N"] == 2 and d["S"] == "ab c  " and d["V"] == [1.0, 0.25]
+
+
+def test_read_al_u_snapshot():
+    s = mpdat.read_dat(DAT)
+    assert s.version == "1.56" and s.n_head == 4 and s.n_tot == 30000
+    assert list(s.species_counts) == [27600, 2400] and s.species_names == ["Al", "U"]
+    np.testing.assert_allclose(s.occupancies, [0.92, 0.08], rtol=1e-6)
+    np.testing.assert_allclose(s.a_par, [79.8159637] * 3, rtol=1e-7)
+    assert np.array_equal(s.a_cell_inv, np.eye(3))
+    assert np.array_equal(s.at_ind[:, 0], np.arange(1, 30001))
+    assert np.array_equal(s.at_ind[:, 3], np.repeat([1, 2], [27600, 2400]))
+    half = s.a_par[0] / 2
+    assert np.all(np.abs(s.at_pos[:, :3]) <= half)
+
+
+def test_sql_geometry_of_the_reference_run():
+    """Q range 20 1/A on the 79.8 A box: 509^3 modes and 2001 bins, as the
+    reference's log reports ("NUFFT_3D requires about 8 Gb")."""
+    s = mpdat.read_dat(DAT)
+    g = sql.geometry(s.a_par, s.a_cell_inv, s.header["N_ROW"], s.n_tot, 20.0, 0.01)
+    assert g.n_qq == (509, 509, 509) and g.n_hist == 2001
+    np.testing.assert_allclose(float(g.dq[0]), 2 * np.pi / 79.8159637, rtol=1e-7)
+    np.testing.assert_allclose(g.at_volume, 79.8159637 ** 3 / 30000, rtol=1e-6)
+
+
+def test_xray_ff_closed_forms():
+    ff = np.load(os.path.join(HERE, "golden", "al_u", "al_u_sql_golden.npz"))["ffpar"]
+    f = sql.xray_ff(ff, np.array([0.0, 4 * np.pi]))
+    np.testing.assert_allclose(f[:, 0], ff[:, [0, 2, 4, 6, 8]].sum(axis=1))        # f(0) = sum a + c
+    s2 = 1.0                                                                       # (Q/4pi)^2 at Q = 4 pi
+    np.testing.assert_allclose(f[:, 1], ff[:, 8] + sum(ff[:, 2 * k] * np.exp(-ff[:, 2 * k + 1] * s2) for k in range(4)))
+
+
+def test_sql_output_faber_ziman_of_an_ideal_gas():
+    """Uncorrelated atoms: every partial histogram equals its asymptote
+    (diagonal occ_i, off-diagonal 0) at every Q, so Z(Q) and F(Q) are 0 for
+    every pair and the total S(Q) is 1, whatever the weights and smoothing."""
+    g = sql.geometry([50.0] * 3, np.eye(3), [1, 1, 1], 1000, 5.0, 0.01)
+    occ = np.array([0.7, 0.3])
+    h = np.zeros((2, 2, g.n_hist))
+    h[0, 0, :], h[1, 1, :] = occ[0], occ[1]
+    ff = np.load(os.path.join(HERE, "golden", "al_u", "al_u_sql_golden.npz"))["ffpar"]
+    q, tot, z = sql.output(h, g, 5, occ, "xray", ffpar=ff, smooth_fwhm=5)
+    assert tot is None and np.abs(z).max() < 1e-12
+    q, tot, s = sql.output(h, g, 4, occ, "xray", ffpar=ff, smooth_fwhm=5)
+    assert np.abs(tot - 1).max() < 1e-12
+    q, tot, f = sql.output(h, g, 3, occ, "xray", ffpar=ff, smooth_fwhm=5)
+    assert np.abs(tot).max() < 1e-12 and np.abs(f).max() < 1e-12
+    q, tot, s = sql.output(h, g, 4, occ, "neutron", b_coh=[3.4, 8.4], smooth_fwhm=1)
+    # without smoothing the first written row is one the reference zeroed
+    # before weighting (mp_sql56.f90:1404 vs i_start at :1814), so it holds
+    # only the asymptote terms
+    assert np.abs(tot[1:] - 1).max() < 1e-12
+    assert abs(q[0] - 0.49) < 1e-6                # first written row: bin int(.5/q_step) + n_smooth/2 (1-based)

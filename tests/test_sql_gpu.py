@@ -1,0 +1,91 @@
+"""mp_sql on the GPU: the |Q| histogram kernel against numpy, and the whole
+path against the reference's own golden tables.
+
+Golden: examples/amorphous_Al92U08/Al_U_large_sql_0001_40..43.txt, computed
+by the reference's mp_sql (FINUFFT 2D slices at eps 1e-6 stacked over k3, FP32
+amplitudes and histograms) from the snapshot in tests/golden/al_u, with X-ray
+weights, Q range 20 1/A, q_step 0.01, smoothing FWHM 5 steps.  Both sides
+carry the NUFFT's eps-level error and the reference its FP32 sums, so the
+comparison is to a relative l2 tolerance per column (stated below), not bit
+for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_sql_hist_kernel_vs_numpy(gpu_lib):
+    import ctypes as C
+    import torch
+    from mp_tools_amd import _lib
+    rng = np.random.default_rng(4)
+    N = (15, 13, 11)
+    nsp = 3
+    nq = int(np.prod(N))
+    a = (rng.standard_normal((nsp, nq)) + 1j * rng.standard_normal((nsp, nq))).astype(np.complex128)
+    dq = np.array([0.31, 0.27, 0.4], np.float32)
+    ainv = np.array([[1.0, 0.1, 0.0], [0.0, 0.9, 0.2], [0.05, 0.0, 1.1]], np.float32)
+    q_range, q_step, n_hist = np.float32(2.5), np.float32(0.05), 51
+    hist = torch.zeros(n_hist * 6, dtype=torch.float64, device="cuda")
+    da = torch.tensor(a, device="cuda")
+    rc = gpu_lib.mpsq_sql_hist(C.c_void_p(da.data_ptr()), nsp, 1, (C.c_int64 * 3)(*N), (C.c_float * 3)(*dq),
+                               (C.c_float * 9)(*ainv.reshape(-1)), float(q_range), float(q_step), n_hist,
+                               C.c_void_p(hist.data_ptr()), None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    got = hist.cpu().numpy().reshape(n_hist, 6)
+    # numpy: the same FP32 |Q| and bin, FP64 products
+    i3, i2, i1 = np.meshgrid(*[np.arange(n) for n in N[::-1]], indexing="ij")
+    u = [((i - n // 2).astype(np.float32) * dq[d]).ravel() for d, (i, n) in enumerate(zip((i1, i2, i3), N))]
+    v = [(ainv[r, 0] * u[0] + ainv[r, 1] * u[1]).astype(np.float32) + ainv[r, 2] * u[2] for r in range(3)]
+    q = np.sqrt((v[0] * v[0] + v[1] * v[1]).astype(np.float32) + v[2] * v[2]).astype(np.float32)
+    ok = q <= q_range
+    b = np.floor(q / q_step + np.float32(0.5)).astype(np.int64)
+    ref = np.zeros((n_hist, 6))
+    p = 0
+    for s in range(nsp):
+        for t in range(s + 1):
+            np.add.at(ref[:, p], b[ok], (a[s] * np.conj(a[t])).real[ok])
+            p += 1
+    assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max()
+    # 64 KB of bins x pairs is the limit
+    rc = gpu_lib.mpsq_sql_hist(C.c_void_p(da.data_ptr()), nsp, 1, (C.c_int64 * 3)(*N), (C.c_float * 3)(*dq),
+                               (C.c_float * 9)(*ainv.reshape(-1)), 100.0, 0.001, 100001,
+                               C.c_void_p(hist.data_ptr()), None)
+    assert rc == _lib.ERR_SIZE
+
+
+@pytest.fixture(scope="module")
+def al_u_hist():
+    from mp_tools_amd import mpdat, sql
+    s = mpdat.read_dat(os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat"))
+    g = sql.geometry(s.a_par, s.a_cell_inv, s.header["N_ROW"], s.n_tot, 20.0, 0.01)
+    tm = {}
+    h = sql.histogram([s.at_pos], s.species_counts, g, eps=1e-6, precision="fp64", timing=tm)
+    print(f"mp_sql Al/U 509^3: {tm['ms']:.1f} ms per snapshot (2 species)")
+    return s, g, h
+
+
+# relative l2 (over the written rows) of each column against the reference
+GOLDEN_TOL = 2e-3
+
+
+@pytest.mark.parametrize("key,mode", [("Z", 5), ("I", 6), ("S", 4), ("Iu", 7)])
+def test_al_u_against_reference_tables(gpu_lib, al_u_hist, key, mode):
+    from mp_tools_amd import sql
+    s, g, h = al_u_hist
+    gold = np.load(os.path.join(HERE, "golden", "al_u", "al_u_sql_golden.npz"))
+    ref = gold[key]
+    q, tot, parts = sql.output(h, g, mode, s.occupancies, "xray", ffpar=gold["ffpar"],
+                               smooth_fwhm=int(gold["smooth_fwhm"]))
+    got = parts if tot is None else np.column_stack([tot, parts])
+    assert got.shape == ref[:, 1:].shape
+    np.testing.assert_allclose(q, ref[:, 0], rtol=1e-6)
+    for c in range(got.shape[1]):
+        err = np.linalg.norm(got[:, c] - ref[:, c + 1]) / np.linalg.norm(ref[:, c + 1])
+        print(f"{key} column {c}: rel l2 vs reference table {err:.2e}")
+        assert err < GOLDEN_TOL
