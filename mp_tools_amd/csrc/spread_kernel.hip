@@ -26,8 +26,12 @@
 // ~8-13 and a 32-bit integer one ~7-9 (measured).  The FP32 form therefore
 // accumulates in fixed point, and the result does not depend on the order
 // of the atomics:
-//  * narrow (k_spread_narrow): 32-bit cells, 8 bytes per complex cell, so a
-//    tile holds twice the rows of a 64-bit one.  A tile's scale 2^q is set
+//  * narrow (k_spread_narrow): 32-bit fixed point for each of re and im,
+//    packed into one 8-byte cell as the integer im * 2^32 + re (re
+//    sign-extended), so a tile holds twice the rows of a 64-bit one and a
+//    contribution is one 64-bit LDS add (~13 cycles) instead of two 32-bit
+//    ones; the low word of the sum is re's sum, the high word minus re's
+//    sign is im's, both exact while below 2^31.  A tile's scale 2^q is set
 //    from its covering bound C -- the sum over the sort bins it visits of
 //    each bin's bound on how many of its points cover any one cell -- so
 //    that C * max|c| * 2^q < 2^30: every cell sum stays below 2^31 in
@@ -122,10 +126,8 @@ __device__ __forceinline__ void overlap(int l, int r0, int b, int w, int n, int 
 // group:
 //   FFT tile      [rows][RSX] complex R, row r at r * RSX (a wide pass uses
 //                 the first G rows);
-//   narrow planes rows in groups of G; group g owns bytes
-//                 [g, g + 1) * G * RSX * 8 and holds its real plane [G][RS]
-//                 then its imaginary plane [G][RS] of int32, inside its own
-//                 FFT rows;
+//   narrow cells  packed 8-byte cells of row r at [r * RSX, r * RSX + RS),
+//                 inside the row's own FFT slot;
 //   wide planes   one pass of G rows: [2][G][RS] 8-byte cells;
 // then the n1 twiddles.
 __host__ __device__ inline int spread_row_stride(int n1, int w) { return (n1 + w + 1) & ~1; }
@@ -467,7 +469,8 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
     const int vw = (NT / 64 - 1 - wv) * 64;
     const int xcd = blockIdx.x & 7;
     const int xbeg = xr[xcd], xend = xr[xcd + 1];
-    int *acc = reinterpret_cast<int *>(smem_raw);
+    // packed cells: im * 2^32 + re (two's complement, mod 2^64), row r at r * RSX
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem_raw);
     C *tile = reinterpret_cast<C *>(smem_raw);
 
     if (threadIdx.x == 0) S.next = xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
@@ -492,13 +495,10 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         const SpreadJob<R> &J = jobs[j];
         const Geo &g = J.g;
         const int n1 = g.n1, w = g.w;
-        const int RS = spread_row_stride(n1, W);
-        const int RSX = spread_row_stride_x(n1, W);
+        const int RSX = spread_row_stride_x(n1, W);  // >= n1 + W: a stencil never leaves its row
         const int by = (g.dim >= 2) ? g.by : 1;
         const int bz = (g.dim >= 3) ? g.bz : 1;
         const int rows = by * bz;
-        const int G = J.grp, gsh = J.grp_sh;
-        const int GS32 = G * RSX * 2;  // int32 cells of one group's region
         const R beta = (R)g.beta, ic = (R)g.ic;
         const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
         const float am = S.amax[pb];
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
             uint4 *a4 = reinterpret_cast<uint4 *>(smem_raw);
             for (int i = threadIdx.x; i < rows * RSX / 2; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
         } else if (threadIdx.x == 0) {
-            const int np = rows >> gsh;
+            const int np = rows >> J.grp_sh;  // wide passes of G rows
             const int k = atomicAdd(wc + CTR_FB_COUNT, np);
             for (int p = 0; p < np; ++p) fbl[k + p] = cur * MAX_WIDE_PASSES + p;
         }
@@ -563,15 +563,15 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                         if (iy < ny) {
                             const int r = rr0 + iy;
                             const R wyz = ky[iy] * wz;
-                            const int cell = (r >> gsh) * GS32 + (r & (G - 1)) * RS + p.lx;
                             const float sr = csx * wyz, si = csy * wyz;
-                            int *ar = acc + cell, *ai = acc + cell + G * RS;
+                            unsigned long long *ac = acc + r * RSX + p.lx;
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
-                                __hip_atomic_fetch_add(ar + m, rint_up(sr * kx[m]), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                                __hip_atomic_fetch_add(ai + m, rint_up(si * kx[m]), __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                                const int re = rint_up(sr * kx[m]), im = rint_up(si * kx[m]);
+                                // im * 2^32 + re with re sign-extended: one 64-bit add per cell
+                                const unsigned long long pk =
+                                    ((unsigned long long)(unsigned)(im + (re >> 31)) << 32) | (unsigned)re;
+                                __hip_atomic_fetch_add(ac + m, pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
                     }
@@ -598,12 +598,11 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
             // the ghost cells x = n1 + i fold onto i < w here (32-bit adds
             // wrap; the final sums are exact); 2^-q goes into the output factor
             auto load_acc = [&](int row, int i) -> C {
-                const int a = (row >> gsh) * GS32 + (row & (G - 1)) * RS + i;
-                int re = acc[a], im = acc[a + G * RS];
-                if (i < W) {
-                    re += acc[a + n1];
-                    im += acc[a + G * RS + n1];
-                }
+                const int a = row * RSX + i;
+                unsigned long long T = acc[a];
+                if (i < W) T += acc[a + n1];
+                const int re = (int)(unsigned)T;                // |sum| < 2^31: exact
+                const int im = (int)(unsigned)(T >> 32) - (re >> 31);  // undo re's sign extension
                 return cmk<C>((float)re, (float)im);
             };
             auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
