@@ -1,0 +1,108 @@
+"""mp_sqom map-file writer (mp_sqom56.f90:1994-2045) against the reference's
+own BZO tables (tests/golden/bzo, extracted by make_bzo_txt.py) and closed
+forms of the Fortran edit descriptors.
+
+The BZO maps' trajectory is not in the reference tree, so the tables pin the
+text layout only: the header is rebuilt from the values it prints and must
+match record by record, and the map records are parsed back to REAL(4) and
+must be rewritten unchanged (rows, groups of five, e10.3)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from mp_tools_amd import mptxt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return json.load(open(os.path.join(HERE, "golden", "bzo", "bzo_sq_tables.json")))
+
+
+# the values the BZO tables print (BZO200d.par / the example's mp_sqom run)
+BZO = dict(mp_tool="MP_SQOM 1.56", subst_name="BZO200d_demo", shell="", file_dat="bzo200d_n0001....0020.dat",
+           t0=np.float32("64.0400009"), t_step=np.float32("4.00009155E-02"), n_row=(32, 32, 32),
+           temp=np.float32("199.586273"), at_names=("Ba", "Zr", "O1", "O2", "O3"), at_occup=(1, 1, 1, 1, 1),
+           weight_scheme="Unit weights", b_coh=(0.507, 0.716, 0.580, 0.580, 0.580), at_mask=(1, 1, 1, 1, 1),
+           atc=(0, 0), t_width=np.float32("0.200004578"), e1=(1, 0, 0), e2=(0, 1, 0), q_center=(2, 2, 0),
+           qx_range=(-0.5, 4.5), qy_range=(-0.5, 4.5))
+
+
+def _parse_rows(rows):
+    return np.array([[np.float32(v) for v in r.split()] for r in rows], dtype=np.float32)
+
+
+@pytest.mark.parametrize("tag", ["07", "11"])
+def test_bzo_header_records(tables, tag):
+    t = tables[tag]
+    kw = dict(BZO)
+    if tag == "11":  # E = 0 map at the example's resolution
+        kw.update(total=False, f_plot=0.0, f_width=np.float32("4.99988556"))
+    cs = np.zeros((t["n_rows"], 160), np.float32)
+    text = mptxt.sqom_table(cs, **kw).split("\n")
+    for k, (got, want) in enumerate(zip(text[:18], t["header"])):
+        # the example's substance record carries the untrimmed 40-character
+        # name (trailing blanks); every other record matches to the byte
+        if k == 2:
+            assert got.rstrip() == want.rstrip(), k
+        else:
+            assert got == want, (k, got, want)
+
+
+@pytest.mark.parametrize("tag", ["07", "11"])
+def test_bzo_map_records_round_trip(tables, tag):
+    t = tables[tag]
+    for rows in (t["first_rows"], t["last_rows"]):
+        a = _parse_rows(rows)
+        assert a.shape == (3, 160)
+        assert mptxt.map_rows(a) == rows
+    assert t["row_lengths"] == [len(mptxt.map_rows(np.ones((1, 160), np.float32))[0])]
+
+
+def test_e10_3_closed_forms():
+    assert mptxt.e10_3(0.0) == " 0.000E+00"
+    assert mptxt.e10_3(1.0) == " 0.100E+01"
+    assert mptxt.e10_3(-1.0) == "-0.100E+01"
+    assert mptxt.e10_3(0.2425e-3) in (" 0.242E-03", " 0.243E-03")  # tie in decimal, not in binary
+    assert mptxt.e10_3(0.24251e-3) == " 0.243E-03"
+    assert mptxt.e10_3(9.9996) == " 0.100E+02"  # rounding carries into the exponent
+    assert mptxt.e10_3(123456.0) == " 0.123E+06"
+    assert mptxt.e10_3(1.4e-45) == " 0.140E-44"  # smallest REAL(4) denormal
+    assert mptxt.e10_3(3.0e38) == " 0.300E+39"
+
+
+def test_list_directed_closed_forms():
+    assert mptxt.ld_record() == ""
+    assert mptxt.ld_record("a", "b") == " ab"  # no separator between character items
+    assert mptxt.ld_record("x", 7) == " x" + " " + " " * 10 + "7"  # separator, then I11
+    assert mptxt.ld_record(np.float32(0.5)) == "  0.500000000    "
+    assert mptxt.ld_record(np.float32(-0.5)) == " -0.500000000    "
+    assert mptxt.ld_record(np.float32(1.0e9)) == "   1.00000000E+09"  # E form at 1e9
+    # E form below 0.1; REAL(4) 0.09 is 0.0900000035762787...
+    assert mptxt.ld_record(np.float32(0.09)) == "   9.00000036E-02"
+    assert mptxt.ld_record(np.float32(999999999.0)) == "   1.00000000E+09"  # rounds to 1e9: E form
+
+
+def test_cs_plot_orientation():
+    # S(Q) in CMCL order, k1 fastest: rows i are k1, columns j are k2
+    n_qx, n_qy = 4, 3
+    s = np.arange(n_qx * n_qy, dtype=np.float64)
+    cs = mptxt.cs_plot_total(s, n_qx, n_qy)
+    assert cs.shape == (4, 3)
+    for i in range(n_qx):
+        for j in range(n_qy):
+            assert cs[i, j] == s[j * n_qx + i]
+
+
+def test_full_table_shape():
+    cs = np.random.default_rng(1).random((7, 12)).astype(np.float32)
+    text = mptxt.sqom_table(cs, **BZO).split("\n")
+    assert text[17] == "" and text[-1] == ""
+    rows = text[18:-1]
+    assert len(rows) == 7
+    assert all(len(r) == 2 * 51 + 1 + 2 * 10 for r in rows)  # two full groups, one of two values
+    back = _parse_rows(rows)
+    assert np.allclose(back, cs, rtol=5e-3)
