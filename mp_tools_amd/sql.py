@@ -226,3 +226,70 @@ def output(sq_hist: np.ndarray, geo: SqlGeometry, mode: int, occ, weighting: str
     rows = slice(i_start, n_hist)
     q = (np.arange(n_hist, dtype=np.float32) * geo.q_step)[rows]
     return q, (None if total is None else total[rows]), np.stack([c[rows] for c in cols], axis=1)
+
+
+# names the text file gives the output modes (pdf_out, mp_sql56.f90:171)
+TEXT_NAMES = {3: "F(Q)", 4: "S(Q)", 5: "Z(Q)", 6: "I(Q)", 7: "I(Q)"}
+
+
+def header_amplitudes(weighting: str, occ, ffpar=None, b_coh=None, scf=None, mixed: bool = True) -> np.ndarray:
+    """at_scf * at_weight as the text header prints them (mp_sql56.f90:1309-1352):
+    the species weight (1, b_coh, or the X-ray form factor at the first Q
+    point, Q = 0), averaged in place over mixed occupancies, REAL(4)."""
+    occ = np.asarray(occ, np.float32)
+    occ = occ / occ.sum()
+    nsp = len(occ)
+    scf = np.ones(nsp, np.float32) if scf is None else np.asarray(scf, np.float32)
+    if weighting == "xray":
+        w = xray_ff(ffpar, np.zeros(1))[:, 0].astype(np.float32)
+    elif weighting == "neutron":
+        w = np.asarray(b_coh, np.float32).copy()
+    else:
+        w = np.ones(nsp, np.float32)
+    avm = np.ones((nsp, nsp), np.float32) if mixed else np.eye(nsp, dtype=np.float32)
+    for j in range(nsp):
+        w[j] = np.float32(np.sum(w * occ * avm[j]) / np.sum(occ * avm[j]))
+    return (scf * w).astype(np.float32)
+
+
+def table_text(q, total, partials, mode: int, *, at_names, occ, amplitudes, q_step: float, smooth_fwhm: int = 1,
+               subst_name: str = "", shell: str = "", mp_tool: str = "MP_SQL 1.56", time_stamp: str = "",
+               file_dat_t0: str = "", file_dat: str = "", nfile_step: int = 1, proc_num: int = 1,
+               n_row=(1, 1, 1), a_par=(1.0, 1.0, 1.0), scf=None, masks=None) -> str:
+    """The text of mp_sql's output file (mp_sql56.f90:1800-1829) for the columns
+    ``output`` returns: the header records, then one list-directed record per
+    Q row (Q, [total,] the partials (1,1), (2,1), (2,2), ...), REAL(4)."""
+    from .mptxt import ld_record, _R4
+
+    nsp = len(at_names)
+    scf = np.ones(nsp) if scf is None else np.asarray(scf)
+    masks = np.ones(nsp, dtype=int) if masks is None else np.asarray(masks, dtype=int)
+    occ = np.asarray(occ, np.float32)
+    occ = occ / occ.sum()
+    rec = [
+        ld_record("Substance:   ", subst_name.strip(), "  ", shell.ljust(10)[:10], "       ",
+                  (mp_tool.strip() + " " + time_stamp.strip())),
+        ld_record("Input files: ", file_dat_t0.strip(), " to ", file_dat.strip(), " step", int(nfile_step)),
+        ld_record("OMP processes  = ", int(proc_num)),
+        ld_record("Supercell size:", [int(v) for v in n_row]),
+        ld_record("Unit cell parameter:", [_R4(v) for v in a_par]),
+        ld_record("Atoms/unit_cell:", int(nsp)),
+        ld_record("Atoms       :", *["    " + str(n).ljust(4)[:4] for n in at_names]),
+        " Atom no.    :" + "".join("%8d" % (i + 1) for i in range(nsp)),
+        " Occupancies: " + "".join("%8.4f" % v for v in occ),
+        " Amplitudes  : " + "".join("%8.4f" % np.float32(v) for v in amplitudes),
+        " Scale factors:" + "".join(" %8.4f" % np.float32(v) for v in scf),
+        " Masks\t      : " + "".join("%8d" % int(v) for v in masks),
+        ld_record("Smoothing FWHM [Å]", _R4(np.float32(smooth_fwhm) * np.float32(q_step))),
+        ld_record(),
+        ld_record("Output: ", TEXT_NAMES[mode]),
+    ]
+    pairs = [(a + 1, b + 1) for a in range(nsp) for b in range(a + 1)]
+    title = "    Q [A-1]          " + ("" if mode == 5 else "total          ")
+    # '(a,50(2i3,11x))': the 11x after the last pair writes nothing
+    rec.append(title + (" " * 11).join("%3d%3d" % p for p in pairs))
+    part = np.asarray(partials, np.float32)
+    for i in range(len(q)):
+        items = [_R4(q[i])] + ([] if mode == 5 else [_R4(total[i])]) + [_R4(v) for v in part[i]]
+        rec.append(ld_record(*items))
+    return "".join(r + "\n" for r in rec)

@@ -8,8 +8,11 @@ Outputs: examples/amorphous_Al92U08/Al_U_large_sql_0001_40..43.txt, the tables
          the reference's mp_sql wrote for that snapshot (log:
          examples/amorphous_Al92U08/mp_sql_cmd_log.txt): Z(Q), I(Q), S(Q) and
          unscaled I(Q), X-ray weights, Q range 20 1/A, q_step 0.01, smoothing
-         FWHM 5 steps.  Stored as float64 arrays in al_u_sql_golden.npz.
+         FWHM 5 steps.  Stored as float64 arrays in al_u_sql_golden.npz; the
+         header records and the first and last three data records of each
+         table as text in al_u_sql_text.json.
 """
+import json
 import os
 import shutil
 
@@ -43,6 +46,14 @@ def main():
          (("Z", 40), ("I", 41), ("S", 42), ("Iu", 43))}
     np.savez_compressed(os.path.join(OUT, "al_u_sql_golden.npz"), ffpar=np.array([ff["Al"], ff["U"]]),
                         q_range=20.0, q_step=0.01, smooth_fwhm=5, **t)
+    text = {}
+    for m, n in (("Z", 40), ("I", 41), ("S", 42), ("Iu", 43)):
+        lines = open(os.path.join(EX, f"Al_U_large_sql_0001_{n}.txt"), encoding="utf-8").read().splitlines()
+        start = next(i for i, l in enumerate(lines) if l.strip().startswith("Q [A-1]")) + 1
+        data = [l for l in lines[start:] if l.strip()]
+        text[m] = {"header": lines[:start], "first_rows": data[:3], "last_rows": data[-3:], "n_rows": len(data)}
+    with open(os.path.join(OUT, "al_u_sql_text.json"), "w", encoding="utf-8") as f:
+        json.dump(text, f, indent=1, ensure_ascii=False)
     for k, v in t.items():
         print(k, v.shape, v[0])
 

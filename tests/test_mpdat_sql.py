@@ -9,6 +9,7 @@ cell matrices, at_ind(1) = serial number, at_ind(4) = species
 import os
 
 import numpy as np
+import pytest
 
 from mp_tools_amd import mpdat, sql
 
@@ -73,3 +74,46 @@ def test_sql_output_faber_ziman_of_an_ideal_gas():
     # only the asymptote terms
     assert np.abs(tot[1:] - 1).max() < 1e-12
     assert abs(q[0] - 0.49) < 1e-6                # first written row: bin int(.5/q_step) + n_smooth/2 (1-based)
+
+
+def _al_u_header_kw(text_header):
+    """The run's parameters as the reference's Al/U tables print them; the
+    time stamp is that table's own."""
+    stamp = " ".join(text_header[0].split()[-2:])
+    return dict(subst_name="Al_U", mp_tool="MP_SQL 1.56", time_stamp=stamp,
+                file_dat_t0="./data/Al_U_large_n0001.dat", file_dat="./data/Al_U_large_n0001.dat",
+                nfile_step=1, proc_num=2, n_row=(1, 1, 1), a_par=(np.float32("79.8159637"),) * 3,
+                at_names=("Al", "U"), occ=(0.92, 0.08), q_step=0.01, smooth_fwhm=5)
+
+
+def _check_header(got, want):
+    # record 1: the example's binary spaced the substance record differently
+    # from the source's (shells(1) is written untrimmed there); compare its words
+    assert got[0].split() == want[0].split()
+    for k in range(1, len(want)):
+        assert got[k] == want[k], (k, got[k], want[k])
+
+
+@pytest.mark.parametrize("key,mode", [("Z", 5), ("I", 6), ("S", 4), ("Iu", 7)])
+def test_sql_text_records_of_the_reference_tables(key, mode):
+    """mp_sql's text file (mp_sql56.f90:1800-1829): the header is rebuilt from
+    the run's parameters (the X-ray amplitudes from the form factors), and the
+    reference's own data records, parsed back to REAL(4), are rewritten
+    unchanged."""
+    import json
+    gd = os.path.join(HERE, "golden", "al_u")
+    text = json.load(open(os.path.join(gd, "al_u_sql_text.json"), encoding="utf-8"))[key]
+    gold = np.load(os.path.join(gd, "al_u_sql_golden.npz"))
+    kw = _al_u_header_kw(text["header"])
+    amp = sql.header_amplitudes("xray", kw["occ"], ffpar=gold["ffpar"])
+    assert ["%8.4f" % a for a in amp] == [" 19.3117", " 25.1244"]
+    ref = gold[key]
+    assert ref.shape[0] == text["n_rows"]
+    for sl, rows in ((slice(0, 3), text["first_rows"]), (slice(-3, None), text["last_rows"])):
+        r = ref[sl]
+        tot = None if mode == 5 else r[:, 1]
+        parts = r[:, 1:] if mode == 5 else r[:, 2:]
+        out = sql.table_text(r[:, 0], tot, parts, mode, amplitudes=amp, **kw).split("\n")
+        hdr = text["header"]
+        _check_header(out[:len(hdr)], hdr)
+        assert out[len(hdr):len(hdr) + 3] == rows

@@ -89,3 +89,19 @@ def test_al_u_against_reference_tables(gpu_lib, al_u_hist, key, mode):
         err = np.linalg.norm(got[:, c] - ref[:, c + 1]) / np.linalg.norm(ref[:, c + 1])
         print(f"{key} column {c}: rel l2 vs reference table {err:.2e}")
         assert err < GOLDEN_TOL
+    # the whole text file from these columns: the reference's header records
+    # and as many data records, each parsing back to the REAL(4) values
+    import json
+    text = json.load(open(os.path.join(HERE, "golden", "al_u", "al_u_sql_text.json"), encoding="utf-8"))[key]
+    hdr = text["header"]
+    amp = sql.header_amplitudes("xray", s.occupancies, ffpar=gold["ffpar"])
+    out = sql.table_text(q, tot, parts, mode, subst_name="Al_U", time_stamp=" ".join(hdr[0].split()[-2:]),
+                         file_dat_t0="./data/Al_U_large_n0001.dat", file_dat="./data/Al_U_large_n0001.dat",
+                         proc_num=2, n_row=s.header["N_ROW"], a_par=s.a_par, at_names=s.species_names,
+                         occ=s.occupancies, amplitudes=amp, q_step=0.01,
+                         smooth_fwhm=int(gold["smooth_fwhm"])).split("\n")
+    assert out[0].split() == hdr[0].split() and out[1:len(hdr)] == hdr[1:]
+    rows = out[len(hdr):-1]
+    assert len(rows) == text["n_rows"]
+    back = np.array([[float(v) for v in r.split()] for r in rows])
+    np.testing.assert_array_equal(back[:, 1:].astype(np.float32), got.astype(np.float32))
