@@ -39,11 +39,17 @@ def _fortran_value(tok: str):
 def parse_namelist(text: str) -> dict:
     """Parse one Fortran namelist record ("&NAME  K=v, K2= 3*1.0, ... /") into
     {key: value or list}; repeat counts (3*x) are expanded."""
+    return parse_namelist_group(text)[1]
+
+
+def parse_namelist_group(text: str):
+    """(group name, {key: value or list}) of one namelist record, keys in
+    the record's order."""
     body = text.strip()
     m = re.match(r"&\s*(\w+)(.*)/", body, re.S)
     if not m:
         raise ValueError("not a namelist record: %r" % body[:40])
-    rest = m.group(2)
+    group, rest = m.group(1).upper(), m.group(2)
     # split at KEY= boundaries, keeping quoted strings intact
     parts = re.split(r"(\b[A-Z_][A-Z0-9_]*\s*=)", rest)
     out = {}
@@ -59,7 +65,7 @@ def parse_namelist(text: str) -> dict:
             else:
                 vals.append(_fortran_value(t))
         out[key] = vals[0] if len(vals) == 1 else vals
-    return out
+    return group, out
 
 
 @dataclass
@@ -69,6 +75,8 @@ class MPSnapshot:
     header: dict = field(default_factory=dict)
     at_ind: np.ndarray = None     # int32 [n_tot, 4]
     at_pos: np.ndarray = None     # float32 [n_tot, 4]
+    time_stamp: str = ""
+    groups: list = field(default_factory=list)  # [(name, {key: value})] in record order
 
     @property
     def n_tot(self) -> int:
@@ -108,14 +116,16 @@ def read_dat(path: str) -> MPSnapshot:
     if len(raw) % rb:
         raise ValueError(f"{path}: size {len(raw)} is not a multiple of the {rb}-byte record")
     rec = lambda i: raw[(i - 1) * rb:i * rb]  # 1-based like the Fortran REC=
-    head = rec(1).decode("latin1")
+    head = rec(1).decode("latin1").rstrip("\x00 ")
     if not head.startswith("MP_TOOLS"):
         raise ValueError(f"{path}: no MP_TOOLS header record")
     f = head.split()
     version, n_head = f[1], int(f[2])
-    hdr = {}
+    hdr, groups = {}, []
     for i in range(2, n_head + 1):
-        hdr.update(parse_namelist(rec(i).decode("latin1").rstrip("\x00 ")))
+        g, d = parse_namelist_group(rec(i).decode("latin1").rstrip("\x00 "))
+        groups.append((g, d))
+        hdr.update(d)
     n_tot, n_rec = int(hdr["N_TOT"]), int(hdr["N_REC"])
     if n_rec * L_REC < 4 * n_tot or (n_rec - 1) * L_REC >= 4 * n_tot:
         raise ValueError(f"{path}: N_REC={n_rec} does not match N_TOT={n_tot} at {L_REC} words per record")
@@ -125,7 +135,123 @@ def read_dat(path: str) -> MPSnapshot:
     ind = np.frombuffer(raw, np.int32, 4 * n_tot, o).reshape(n_tot, 4).copy()
     o += n_rec * rb
     pos = np.frombuffer(raw, np.float32, 4 * n_tot, o).reshape(n_tot, 4).copy()
-    s = MPSnapshot(version=version, n_head=n_head, header=hdr, at_ind=ind, at_pos=pos)
+    s = MPSnapshot(version=version, n_head=n_head, header=hdr, at_ind=ind, at_pos=pos,
+                   time_stamp=" ".join(f[3:]), groups=groups)
     if int(s.species_counts.sum()) != n_tot:
         raise ValueError(f"{path}: NSUPER_R sums to {s.species_counts.sum()}, N_TOT is {n_tot}")
     return s
+
+
+# ---------------------------------------------------------------------------
+# writer (code/src/mp_lbin56.f90:1436-1480): record 1 is
+# dat_source(16) // version(4) // string(40) // '   ' // time_stamp with
+# string the list-directed n_head; records 2..n_head each hold one namelist
+# group written by gfortran into a 1024-character internal record; every
+# record is 4 * L_REC bytes, the part not written left zero.
+# ---------------------------------------------------------------------------
+
+def _nml_value(v, lead: bool) -> str:
+    """One namelist value as gfortran writes it: integers left-justified in
+    11 columns, REAL(4) as G16.9 (left-justified in 16 after a repeat
+    count), character with quote delimiters and its full length."""
+    from .mptxt import _g16_9
+    if isinstance(v, str):
+        return '"' + v + '"'
+    if isinstance(v, (int, np.integer)) and not isinstance(v, bool):
+        return ("%d" % int(v)).ljust(11)
+    f = _g16_9(v)
+    return f if lead else f.lstrip(" ").ljust(16)
+
+
+def namelist_record(group: str, items: dict) -> str:
+    """gfortran's ``write(internal, nml=group)`` into one record: '&NAME',
+    then per object ' ' (record separator) + ' NAME=' + values, each value
+    followed by ','; equal consecutive values as ' r*value'; after every
+    fifth value of an object two more blanks; finally '  /'."""
+    out = ["&" + group.upper()]
+    for key, val in items.items():
+        out.append("  " + key.upper() + "=")
+        vals = val if isinstance(val, list) else [val]
+        i = nv = 0
+        while i < len(vals):
+            j = i
+            while j + 1 < len(vals) and type(vals[j + 1]) is type(vals[i]) and _same(vals[j + 1], vals[i]):
+                j += 1
+            r = j - i + 1
+            if r > 1:
+                out.append(" %d*" % r + _nml_value(vals[i], lead=False))
+            else:
+                out.append(_nml_value(vals[i], lead=True))
+            out.append(",")
+            nv += 1
+            if nv % 5 == 0:
+                out.append("  ")
+            i = j + 1
+    out.append("  /")
+    return "".join(out)
+
+
+def _same(a, b) -> bool:
+    if isinstance(a, float):
+        return np.float32(a).tobytes() == np.float32(b).tobytes()
+    return a == b
+
+
+def _rec(data: bytes) -> bytes:
+    rb = 4 * L_REC
+    if len(data) > rb:
+        raise ValueError("record longer than %d bytes" % rb)
+    return data + b"\x00" * (rb - len(data))
+
+
+def _array_records(a: np.ndarray) -> bytes:
+    flat = np.ascontiguousarray(a).reshape(-1).tobytes()
+    n_rec = -(-len(flat) // (4 * L_REC))
+    return flat + b"\x00" * (n_rec * 4 * L_REC - len(flat))
+
+
+def dat_bytes(snap: MPSnapshot) -> bytes:
+    """The bytes of an MP_TOOLS snapshot file holding snap's header groups,
+    AT_IND and AT_POS (no velocity / force / shell records)."""
+    if int(snap.header.get("N_TRAJ", 0)) != 0 or int(snap.header.get("J_SHELL_OUT", 0)) == 1:
+        raise ValueError("velocities, forces and shells are not written")
+    n_head = 1 + len(snap.groups)
+    head = ("MP_TOOLS".ljust(16) + snap.version.ljust(4)[:4] + (" " + "%11d" % n_head).ljust(40)
+            + "   " + snap.time_stamp.strip()).ljust(L_REC)
+    out = [_rec(head.encode("latin1"))]
+    for g, d in snap.groups:
+        out.append(_rec(namelist_record(g, d).ljust(L_REC).encode("latin1")))
+    out.append(_array_records(np.asarray(snap.at_ind, np.int32)))
+    out.append(_array_records(np.asarray(snap.at_pos, np.float32)))
+    return b"".join(out)
+
+
+def write_dat(path: str, snap: MPSnapshot) -> None:
+    with open(path, "wb") as f:
+        f.write(dat_bytes(snap))
+
+
+def load_trajectory(paths, device="cuda", pin: bool = True):
+    """Read MP_TOOLS snapshots (one per frame) into one device tensor
+    float32 [T, n_tot, 4] (x, y, z, charge; species-major atom order), the
+    layout the S(Q,w) engine takes.  Frames are staged through one pinned
+    host buffer and copied with one transfer.  All files must hold the same
+    atoms (N_TOT and NSUPER_R)."""
+    import torch
+
+    paths = list(paths)
+    if not paths:
+        raise ValueError("no snapshot files")
+    first = read_dat(paths[0])
+    n_tot, counts = first.n_tot, first.species_counts
+    host = torch.empty((len(paths), n_tot, 4), dtype=torch.float32,
+                       pin_memory=bool(pin and torch.cuda.is_available()))
+    hv = host.numpy()
+    hv[0] = first.at_pos
+    for k, p in enumerate(paths[1:], start=1):
+        s = read_dat(p)
+        if s.n_tot != n_tot or not np.array_equal(s.species_counts, counts):
+            raise ValueError(f"{p}: atoms differ from {paths[0]}")
+        hv[k] = s.at_pos
+    return host.to(device, non_blocking=False), first
+

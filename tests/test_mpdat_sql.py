@@ -117,3 +117,58 @@ def test_sql_text_records_of_the_reference_tables(key, mode):
         hdr = text["header"]
         _check_header(out[:len(hdr)], hdr)
         assert out[len(hdr):len(hdr) + 3] == rows
+
+
+def test_dat_writer_reproduces_the_reference_snapshot():
+    """The .dat writer (mp_lbin56.f90:1436-1480, gfortran namelist records)
+    rewrites the reference's own Al/U snapshot byte for byte."""
+    p = os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat")
+    s = mpdat.read_dat(p)
+    assert s.time_stamp == "2023/11/28 10:18:08"
+    assert [g for g, _ in s.groups] == ["DATA_HEADER_1", "DATA_HEADER_2", "DATA_HEADER_3"]
+    assert mpdat.dat_bytes(s) == open(p, "rb").read()
+
+
+def test_namelist_record_closed_forms():
+    rec = mpdat.namelist_record("grp", {"A": [1.0, 1.0, 2.0], "N": 7, "S": ["ab", "ab"], "C": "x "})
+    assert rec == '&GRP  A= 2*1.00000000      ,  2.00000000    ,  N=7          ,  S= 2*"ab",  C="x ",  /'
+    # five values of one object, then two more blanks
+    rec = mpdat.namelist_record("g", {"V": [1, 2, 3, 4, 5, 6]})
+    assert rec == "&G  V=" + ",".join(("%d" % v).ljust(11) for v in range(1, 6)) + ",  " + "6".ljust(11) + ",  /"
+    g, d = mpdat.parse_namelist_group(rec)
+    assert g == "G" and d["V"] == [1, 2, 3, 4, 5, 6]
+
+
+def test_dat_round_trip_and_trajectory_loading(tmp_path):
+    """write -> read of a snapshot whose arrays end inside a record, and the
+    frame stack of the trajectory loader (species-major float4 per frame)."""
+    s = mpdat.read_dat(os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat"))
+    n = 1000  # 4000 words: the last record is partly padding
+    rng = np.random.default_rng(3)
+    s2 = mpdat.MPSnapshot(version=s.version, n_head=s.n_head, time_stamp="2026/01/01 00:00:00",
+                          groups=[(g, dict(d)) for g, d in s.groups],
+                          at_ind=s.at_ind[:n].copy(), at_pos=rng.random((n, 4), dtype=np.float32))
+    d1 = dict(s2.groups[0][1])
+    d1.update(N_TOT=n, N_REC=-(-4 * n // mpdat.L_REC))
+    d2 = dict(s2.groups[1][1])
+    d2.update(NSUPER_R=[900, 100])
+    s2.groups = [("DATA_HEADER_1", d1), ("DATA_HEADER_2", d2), s2.groups[2]]
+    paths = []
+    for k in range(3):
+        p = str(tmp_path / f"f{k}.dat")
+        s2.at_pos = (rng.random((n, 4)) * (k + 1)).astype(np.float32)
+        mpdat.write_dat(p, s2)
+        back = mpdat.read_dat(p)
+        assert back.n_tot == n and list(back.species_counts) == [900, 100]
+        np.testing.assert_array_equal(back.at_pos, s2.at_pos)
+        np.testing.assert_array_equal(back.at_ind, s2.at_ind)
+        assert os.path.getsize(p) == (4 + 2 * d1["N_REC"]) * 4 * mpdat.L_REC
+        paths.append((p, s2.at_pos.copy()))
+    traj, first = mpdat.load_trajectory([p for p, _ in paths], device="cpu")
+    assert tuple(traj.shape) == (3, n, 4)
+    for k, (_, pos) in enumerate(paths):
+        np.testing.assert_array_equal(traj[k].numpy(), pos)
+    other = str(tmp_path / "other.dat")
+    mpdat.write_dat(other, s)
+    with pytest.raises(ValueError):
+        mpdat.load_trajectory([paths[0][0], other], device="cpu")

@@ -105,3 +105,37 @@ def test_al_u_against_reference_tables(gpu_lib, al_u_hist, key, mode):
     assert len(rows) == text["n_rows"]
     back = np.array([[float(v) for v in r.split()] for r in rows])
     np.testing.assert_array_equal(back[:, 1:].astype(np.float32), got.astype(np.float32))
+
+
+@pytest.mark.gpu
+def test_load_trajectory_to_device_and_engine(gpu_lib, tmp_path):
+    """Snapshots -> one device tensor [T, n_tot, 4] -> the S(Q,w) engine's
+    amplitudes, against the same frames passed from host arrays."""
+    import torch
+    from mp_tools_amd import mpdat
+    from mp_tools_amd.sqw import SqwEngine
+    src = os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat")
+    s = mpdat.read_dat(src)
+    rng = np.random.default_rng(5)
+    paths, frames = [], []
+    for k in range(3):
+        pos = s.at_pos.copy()
+        pos[:, :3] += rng.normal(0, 0.05, (s.n_tot, 3)).astype(np.float32)
+        snap = mpdat.MPSnapshot(version=s.version, n_head=s.n_head, header=s.header, at_ind=s.at_ind,
+                                at_pos=pos, time_stamp=s.time_stamp, groups=s.groups)
+        p = str(tmp_path / f"t{k}.dat")
+        mpdat.write_dat(p, snap)
+        paths.append(p)
+        frames.append(pos)
+    traj, first = mpdat.load_trajectory(paths, device="cuda")
+    assert traj.is_cuda and tuple(traj.shape) == (3, s.n_tot, 4)
+    ref = torch.from_numpy(np.stack(frames)).cuda()
+    assert torch.equal(traj, ref)
+    L = float(s.a_par[0])
+    eng = SqwEngine(3, (32, 32, 8), s.species_counts, eps=1e-6, precision="fp32",
+                    weights=[1.0, 1.0], tpe=(2 * np.pi / L,) * 3)
+    a = eng.amplitudes(traj)
+    b = eng.amplitudes(ref)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.isfinite(torch.view_as_real(a)).all()
+    eng.destroy()
