@@ -12,9 +12,9 @@
 //      is spread -- there is no block-wide synchronisation in this loop.  A
 //      lane computes the rows of the tile its stencil covers and the kernel
 //      values phi(d + m) (registers) and adds its w cells per covered row
-//      with LDS atomics into planar accumulator rows (real plane, imaginary
-//      plane) that carry w ghost cells past x = n1, so a stencil never wraps
-//      inside the loop and every cell address is base + constant;
+//      with LDS atomics into accumulator rows (fixed point, see below) that
+//      carry w ghost cells past x = n1, so a stencil never wraps inside the
+//      loop and every cell address is base + constant;
 //   3. runs the x FFT of its rows in LDS (the first Stockham stage reads the
 //      accumulators and folds the ghost cells back onto x < w, rows padded by
 //      one element per 16 against bank conflicts, twiddles in LDS), keeps the
@@ -83,12 +83,17 @@ __device__ __forceinline__ int posmod(int a, int n)
 
 // phi(t) = exp(beta (sqrt(1 - ic t^2) - 1)), |t| <= w/2.  FP32: hardware
 // exp2 / sqrt (relative error ~1e-6 only where phi < 1e-6, i.e. absolute
-// error < 1e-12); FP64: the accurate library functions.
-__device__ __forceinline__ float es_kernel(float t, float beta, float ic)
+// error < 1e-12) with bl = beta * log2(e), so that the exponent is one fma;
+// FP64: the accurate library functions with bl = beta.
+__device__ __forceinline__ float es_kernel(float t, float bl, float ic)
 {
     const float a = 1.0f - t * t * ic;
-    return (a > 0.0f) ? __expf(beta * (__builtin_amdgcn_sqrtf(a) - 1.0f)) : 0.0f;
+    return (a > 0.0f) ? __builtin_amdgcn_exp2f(fmaf(__builtin_amdgcn_sqrtf(a), bl, -bl)) : 0.0f;
 }
+// the kernel's shape argument: beta * log2(e) for FP32, beta for FP64
+template <typename R> __host__ __device__ inline R es_shape(double beta);
+template <> __host__ __device__ inline float es_shape<float>(double beta) { return (float)(beta * 1.4426950408889634); }
+template <> __host__ __device__ inline double es_shape<double>(double beta) { return beta; }
 __device__ __forceinline__ double es_kernel(double t, double beta, double ic)
 {
     const double a = 1.0 - t * t * ic;
@@ -499,7 +504,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         const int by = (g.dim >= 2) ? g.by : 1;
         const int bz = (g.dim >= 3) ? g.bz : 1;
         const int rows = by * bz;
-        const R beta = (R)g.beta, ic = (R)g.ic;
+        const R beta = es_shape<R>(g.beta), ic = (R)g.ic;
         const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
         const float am = S.amax[pb];
         const int cb = S.cb[pb];
@@ -546,14 +551,15 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 R kx[W];
 #pragma unroll
                 for (int m = 0; m < W; ++m) kx[m] = es_kernel(p.dx + (R)m, beta, ic);
+                const R dy0 = p.dy + (R)my, dz0 = p.dz + (R)mz;  // offsets of the first covered rows
                 R ky[W];
 #pragma unroll
                 for (int iy = 0; iy < W; ++iy)
-                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(p.dy + (R)(my + iy), beta, ic) : (R)0) : (R)1;
+                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(dy0 + (R)iy, beta, ic) : (R)0) : (R)1;
                 R kz[W];
 #pragma unroll
                 for (int iz = 0; iz < W; ++iz)
-                    kz[iz] = (g.dim >= 3) ? ((iz < nz) ? es_kernel(p.dz + (R)(mz + iz), beta, ic) : (R)0) : (R)1;
+                    kz[iz] = (g.dim >= 3) ? ((iz < nz) ? es_kernel(dz0 + (R)iz, beta, ic) : (R)0) : (R)1;
                 const float csx = ldexpf(cc.x, q), csy = ldexpf(cc.y, q);  // strength * 2^q
                 for (int iz = 0; iz < nz; ++iz) {
                     const R wz = kz[iz];  // iz is uniform over the lanes still in the loop
@@ -689,7 +695,7 @@ __global__ __launch_bounds__(512, 4) void k_spread_wide(const SpreadJob<R> *__re
         const int G = J.grp;
         const int r0 = pass * G;    // tile rows [r0, r0 + G) of this item
         const int wplane = G * RS;  // cells of one plane
-        const R beta = (R)g.beta, ic = (R)g.ic;
+        const R beta = es_shape<R>(g.beta), ic = (R)g.ic;
         const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
         const float am = S.amax[pb];
 

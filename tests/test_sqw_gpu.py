@@ -108,6 +108,8 @@ RED_CASES = [
     (20, 9, 0, 0, 5, True, False),       # sliding windows (reference default shape, odd n_int)
     (21, 10, 1, -3, 7, False, True),
     (12, 1, 0, 0, 1, True, False),       # n_int = 1: window 1
+    (512, 512, 0, 0, 257, True, False),   # one window over 512 frames (the C2 shape)
+    (256, 256, 1, -100, 200, False, True),  # 256 frames, negative m_lo, per-Q factor
 ]
 
 
@@ -116,7 +118,7 @@ RED_CASES = [
 def test_reduce_vs_oracle(gpu_lib, precision, T, n_int, kind, m_lo, n_f, sub, useq):
     import torch
     from mp_tools_amd.sqw import SqwEngine
-    nm = (16, 12)
+    nm = (16, 12) if T < 256 else (15, 13)  # 195 Q: the last group of 16 columns is ragged
     NQ = int(np.prod(nm))
     rng = np.random.default_rng(T * 100 + n_int)
     # amplitudes with a strong time-constant part (Bragg-like) plus fluctuations
