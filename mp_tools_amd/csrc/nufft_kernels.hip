@@ -247,14 +247,21 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t *sh, int64
     return wbase + x - v;
 }
 
-__global__ __launch_bounds__(256) void k_scan_partial(const int *__restrict__ H, int64_t n, int per_thread,
+// Block b of the scans owns elements [b * SCAN_B, (b + 1) * SCAN_B).
+constexpr int SCAN_PER = 16, SCAN_B = 256 * SCAN_PER;
+
+// total of each block (any order: coalesced loads)
+__global__ __launch_bounds__(256) void k_scan_partial(const int *__restrict__ H, int64_t n,
                                                       int64_t *__restrict__ bsum)
 {
     __shared__ int64_t sh[4];
-    const int64_t base = ((int64_t)blockIdx.x * 256 + threadIdx.x) * per_thread;
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
     int64_t s = 0;
-    for (int k = 0; k < per_thread; ++k)
-        if (base + k < n) s += H[base + k];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int64_t i = b0 + k * 256 + threadIdx.x;
+        if (i < n) s += H[i];
+    }
     int64_t tot;
     block_excl_scan(s, sh, tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
@@ -274,21 +281,39 @@ __global__ __launch_bounds__(256) void k_scan_top(int64_t *__restrict__ bsum, in
     }
 }
 
-__global__ __launch_bounds__(256) void k_scan_final(const int *__restrict__ H, int64_t n, int per_thread,
+// exclusive scan of a block's SCAN_B counts: coalesced loads into LDS,
+// thread t scans elements [16 t, 16 t + 16), coalesced stores from LDS
+// (LDS index i + i / 16 keeps the per-thread runs off one bank)
+__global__ __launch_bounds__(256) void k_scan_final(const int *__restrict__ H, int64_t n,
                                                     const int64_t *__restrict__ bsum, int64_t *__restrict__ Hs)
 {
     __shared__ int64_t sh[4];
-    const int64_t base = ((int64_t)blockIdx.x * 256 + threadIdx.x) * per_thread;
+    __shared__ int cnt[SCAN_B + SCAN_B / 16];
+    __shared__ int64_t out[SCAN_B + SCAN_B / 16];
+    const int64_t b0 = (int64_t)blockIdx.x * SCAN_B;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int i = k * 256 + threadIdx.x;
+        cnt[i + (i >> 4)] = (b0 + i < n) ? H[b0 + i] : 0;
+    }
+    __syncthreads();
+    const int j0 = threadIdx.x * SCAN_PER, l0 = j0 + (j0 >> 4);  // 16 elements: one padded run
     int64_t s = 0;
-    for (int k = 0; k < per_thread; ++k)
-        if (base + k < n) s += H[base + k];
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) s += cnt[l0 + k];
     int64_t tot;
     int64_t run = bsum[blockIdx.x] + block_excl_scan(s, sh, tot);
-    for (int k = 0; k < per_thread; ++k)
-        if (base + k < n) {
-            Hs[base + k] = run;
-            run += H[base + k];
-        }
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        out[l0 + k] = run;
+        run += cnt[l0 + k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+        const int i = k * 256 + threadIdx.x;
+        if (b0 + i < n) Hs[b0 + i] = out[i + (i >> 4)];
+    }
 }
 
 // Stable scatter.  Each block owns one chunk and walks it in rounds of
@@ -562,15 +587,13 @@ void launch_sort_count(const int *key, const SortChunk *chunks, int nchunks, int
 void launch_scan(const int *H, int64_t n, int64_t *bsum, int64_t *Hs, hipStream_t st)
 {
     if (n == 0) return;
-    const int per = 16;
-    const int64_t per_block = 256LL * per;
-    const int nb = (int)((n + per_block - 1) / per_block);
-    hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, st, H, n, per, bsum);
+    const int nb = (int)((n + SCAN_B - 1) / SCAN_B);
+    hipLaunchKernelGGL(k_scan_partial, dim3(nb), dim3(256), 0, st, H, n, bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, st, bsum, nb);
-    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, st, H, n, per, bsum, Hs);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(256), 0, st, H, n, bsum, Hs);
 }
 
-int64_t scan_bsum_size(int64_t n) { return (n + 256LL * 16 - 1) / (256LL * 16) + 1; }
+int64_t scan_bsum_size(int64_t n) { return (n + SCAN_B - 1) / SCAN_B + 1; }
 
 template <typename R>
 void launch_sort_scatter(const int *key, const SortChunk *chunks, int nchunks, int max_nbins, const int64_t *Hs,
