@@ -1097,7 +1097,7 @@ template <typename R> struct Engine {
                 rpT = rp;
             }
             int cols = 1;
-            while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= 64) cols *= 2;
+            while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS) cols *= 2;
             const int ev = tm.begin(ST_REDUCE, st);
             launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, st);
             tm.end(ev, st);

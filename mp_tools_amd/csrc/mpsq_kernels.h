@@ -140,6 +140,9 @@ template <typename R> struct ColJob {
 };
 
 // S(Q,w) reduction parameters
+// Q columns of one k_sqw_single workgroup: a power of two <= this
+constexpr int SQW_MAX_COLS = 32;
+
 struct SqwParams {
     int64_t T, NQ;
     int n_int;

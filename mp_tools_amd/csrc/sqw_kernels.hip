@@ -29,8 +29,8 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     C *tile = reinterpret_cast<C *>(smem_raw);
     C *tw_l = tile + (int)p.T * cols;  // twiddles in LDS
     for (int i = threadIdx.x; i < (int)p.T; i += NT) tw_l[i] = tw[i];
-    __shared__ double red[3 * NT];
-    __shared__ double mean_r[64], mean_i[64];
+    __shared__ double red[NT / 64][SQW_MAX_COLS][3];  // per wave: partial sums per column
+    __shared__ double mean_r[SQW_MAX_COLS], mean_i[SQW_MAX_COLS];
     const int T = (int)p.T;
     const int64_t q0 = (int64_t)blockIdx.x * cols;
     const int ncol = (int)min((int64_t)cols, p.NQ - q0);
@@ -51,25 +51,35 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     }
     __syncthreads();
     {
+        // thread (c, seg) sums frames seg, seg + nseg, ...; the lanes of a
+        // wave that share c (64 / cols of them) are combined by shuffles,
+        // then the waves' partials in LDS
         const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
         double sr = 0.0, si = 0.0, ss = 0.0;
-        if (seg < nseg)
-            for (int t = seg; t < T; t += nseg) {
-                const C v = tile[t * cols + c];
-                sr += (double)v.x;
-                si += (double)v.y;
-                ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-            }
-        red[threadIdx.x] = sr;
-        red[NT + threadIdx.x] = si;
-        red[2 * NT + threadIdx.x] = ss;
+        for (int t = seg; t < T; t += nseg) {
+            const C v = tile[t * cols + c];
+            sr += (double)v.x;
+            si += (double)v.y;
+            ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+        for (int o = cols; o < 64; o <<= 1) {
+            sr += __shfl_xor(sr, o, 64);
+            si += __shfl_xor(si, o, 64);
+            ss += __shfl_xor(ss, o, 64);
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane < cols) {
+            red[wv][lane][0] = sr;
+            red[wv][lane][1] = si;
+            red[wv][lane][2] = ss;
+        }
         __syncthreads();
         if (threadIdx.x < cols) {
             double a = 0.0, b = 0.0, q = 0.0;
-            for (int k = 0; k < nseg; ++k) {
-                a += red[k * cols + threadIdx.x];
-                b += red[NT + k * cols + threadIdx.x];
-                q += red[2 * NT + k * cols + threadIdx.x];
+            for (int k = 0; k < NT / 64; ++k) {
+                a += red[k][threadIdx.x][0];
+                b += red[k][threadIdx.x][1];
+                q += red[k][threadIdx.x][2];
             }
             mean_r[threadIdx.x] = p.subtract_mean ? a / (double)T : 0.0;
             mean_i[threadIdx.x] = p.subtract_mean ? b / (double)T : 0.0;
@@ -230,7 +240,7 @@ void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double
 }
 
 template <typename R> struct SqwTiles;
-template <> struct SqwTiles<float> { static constexpr int E = 32, NT = 256; };
+template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
 template <> struct SqwTiles<double> { static constexpr int E = 16, NT = 256; };
 
 template <typename R> int sqw_tile_elems() { return SqwTiles<R>::E * SqwTiles<R>::NT; }
