@@ -546,12 +546,13 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
 // explicit instantiations + launch helpers
 // ---------------------------------------------------------------------------
 template <typename R> struct Tiles;
-// y pass: 512 threads (16 columns of 512 rows in FP32: 128-byte row
-// segments); final pass: 256 threads and half the tile, so that four
-// workgroups share a CU while each walks its species one after another
-template <> struct Tiles<float> { static constexpr int E = 16; };
-template <> struct Tiles<double> { static constexpr int E = 8; };
-constexpr int colfft_nt(bool final) { return final ? 256 : 512; }
+// y pass: 1024 threads of EY elements (FP32: 16 columns of 512 rows,
+// 128-byte row segments; two workgroups, 32 waves, per CU); final pass:
+// 256 threads of EF elements, half the tile, so that four workgroups share
+// a CU while each walks its species one after another
+template <> struct Tiles<float> { static constexpr int EY = 8, EF = 16; };
+template <> struct Tiles<double> { static constexpr int EY = 4, EF = 8; };
+constexpr int colfft_nt(bool final) { return final ? 256 : 1024; }
 
 
 template <typename R>
@@ -643,7 +644,10 @@ void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t
                        reinterpret_cast<unsigned *>(amax));
 }
 
-template <typename R> int colfft_tile_elems(bool final) { return Tiles<R>::E * colfft_nt(final); }
+template <typename R> int colfft_tile_elems(bool final)
+{
+    return (final ? Tiles<R>::EF : Tiles<R>::EY) * colfft_nt(final);
+}
 
 // LDS of one column pass of a job: tile + twiddles (+ species sum for the final pass)
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
@@ -656,20 +660,20 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
                         hipStream_t st)
 {
     if (nblocks <= 0) return;
-    constexpr int E = Tiles<R>::E, NTF = colfft_nt(true), NTY = colfft_nt(false);
+    constexpr int EF = Tiles<R>::EF, EY = Tiles<R>::EY, NTF = colfft_nt(true), NTY = colfft_nt(false);
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTF, true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, EF, NTF, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, E, NTY, false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, EY, NTY, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         attr = true;
     }
     if (final)
-        hipLaunchKernelGGL((k_colfft<R, E, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
+        hipLaunchKernelGGL((k_colfft<R, EF, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
                            njobs);
     else
-        hipLaunchKernelGGL((k_colfft<R, E, NTY, false>), dim3((unsigned)nblocks), dim3(NTY), lds, st, jobs, blk0s,
+        hipLaunchKernelGGL((k_colfft<R, EY, NTY, false>), dim3((unsigned)nblocks), dim3(NTY), lds, st, jobs, blk0s,
                            njobs);
 }
 
