@@ -548,11 +548,11 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
 template <typename R> struct Tiles;
 // y pass: 1024 threads of EY elements (FP32: 16 columns of 512 rows,
 // 128-byte row segments; two workgroups, 32 waves, per CU); final pass:
-// 256 threads of EF elements, half the tile, so that four workgroups share
-// a CU while each walks its species one after another
-template <> struct Tiles<float> { static constexpr int EY = 8, EF = 16; };
-template <> struct Tiles<double> { static constexpr int EY = 4, EF = 8; };
-constexpr int colfft_nt(bool final) { return final ? 256 : 1024; }
+// 512 threads of EF elements, half the tile, so that three workgroups
+// share a CU while each walks its species one after another
+template <> struct Tiles<float> { static constexpr int EY = 8, EF = 8; };
+template <> struct Tiles<double> { static constexpr int EY = 4, EF = 4; };
+constexpr int colfft_nt(bool final) { return final ? 512 : 1024; }
 
 
 template <typename R>
