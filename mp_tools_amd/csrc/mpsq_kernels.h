@@ -141,7 +141,7 @@ template <typename R> struct ColJob {
 
 // S(Q,w) reduction parameters
 // Q columns of one k_sqw_single workgroup: a power of two <= this
-constexpr int SQW_MAX_COLS = 32;
+constexpr int SQW_MAX_COLS = 64;
 
 struct SqwParams {
     int64_t T, NQ;

@@ -28,8 +28,8 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *tile = reinterpret_cast<C *>(smem_raw);
     C *tw_l = tile + (int)p.T * cols;  // twiddles in LDS
+    double *red = reinterpret_cast<double *>(tw_l + (int)p.T);  // [NT / 64][cols][3]: per-wave column sums
     for (int i = threadIdx.x; i < (int)p.T; i += NT) tw_l[i] = tw[i];
-    __shared__ double red[NT / 64][SQW_MAX_COLS][3];  // per wave: partial sums per column
     __shared__ double mean_r[SQW_MAX_COLS], mean_i[SQW_MAX_COLS];
     const int T = (int)p.T;
     const int64_t q0 = (int64_t)blockIdx.x * cols;
@@ -69,17 +69,17 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         }
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         if (lane < cols) {
-            red[wv][lane][0] = sr;
-            red[wv][lane][1] = si;
-            red[wv][lane][2] = ss;
+            red[(wv * cols + lane) * 3 + 0] = sr;
+            red[(wv * cols + lane) * 3 + 1] = si;
+            red[(wv * cols + lane) * 3 + 2] = ss;
         }
         __syncthreads();
         if (threadIdx.x < cols) {
             double a = 0.0, b = 0.0, q = 0.0;
             for (int k = 0; k < NT / 64; ++k) {
-                a += red[k][threadIdx.x][0];
-                b += red[k][threadIdx.x][1];
-                q += red[k][threadIdx.x][2];
+                a += red[(k * cols + threadIdx.x) * 3 + 0];
+                b += red[(k * cols + threadIdx.x) * 3 + 1];
+                q += red[(k * cols + threadIdx.x) * 3 + 2];
             }
             mean_r[threadIdx.x] = p.subtract_mean ? a / (double)T : 0.0;
             mean_i[threadIdx.x] = p.subtract_mean ? b / (double)T : 0.0;
@@ -252,7 +252,7 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 {
     constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
     const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
-    const size_t lds = (size_t)p.T * (cols + 1) * sizeof(typename cplx<R>::T);
+    const size_t lds = (size_t)p.T * (cols + 1) * sizeof(typename cplx<R>::T) + sizeof(double) * 3 * (NT / 64) * cols;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqw_single<R, E, NT>),
