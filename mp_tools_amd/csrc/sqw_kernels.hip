@@ -241,7 +241,7 @@ void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double
 
 template <typename R> struct SqwTiles;
 template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
-template <> struct SqwTiles<double> { static constexpr int E = 16, NT = 256; };
+template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };
 
 template <typename R> int sqw_tile_elems() { return SqwTiles<R>::E * SqwTiles<R>::NT; }
 
