@@ -294,3 +294,58 @@ def autotune_tiles(pos, dim, n_modes, species_counts, **kw):
     _lib.check("autotune_tiles", L.mpsq_autotune_tiles(C.byref(s), pos.data_ptr(), pos.shape[0],
                                                       C.byref(by), C.byref(bz), C.byref(ms)))
     return by.value, bz.value, ms.value
+
+
+def qplane(n_row, bz_n: float, e1=(1, 0, 0), e2=(0, 1, 0), q_center=(0.0, 0.0, 0.0), a_cell_inv=None,
+           a_par=None, nsuper: int = 2) -> dict:
+    """The reference's Q-plane set-up (mp_sqom56.f90:800-857, 884-907), in its
+    REAL(4) arithmetic: e1, e2 (normalised) and their normal ev, the NUFFT
+    phase scales tpe = 2 pi / (e . (pos_max - pos_min)) with the box
+    [-n_row/2, n_row/2) (lattice data) or [-a_par/2, a_par/2) (one bulk cell,
+    nsuper == 1), the Brillouin-zone counts bz_nx, bz_ny (widened for
+    oblique reciprocal axes), the pixel counts n_qx, n_qy (truncated, then
+    made even) and the plot ranges qx/qy_min, max.  ``engine_kw`` holds the
+    matching SqwEngine arguments (axes, tpe, q0) and ``n_modes``."""
+    f = np.float32
+    twopi = 2.0 * np.pi  # REAL(8) in the reference (mp_sqom56.f90:64); the rest is REAL(4)
+    e1 = np.asarray(e1, f)
+    e2 = np.asarray(e2, f)
+    ev = np.array([e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]], f)
+    n1, n2, nv = (f(np.sqrt(np.dot(v, v))) for v in (e1, e2, ev))
+    e1, e2, ev = e1 / n1, e2 / n2, ev / nv
+    if nsuper == 1:
+        pmin, pmax = -np.asarray(a_par, f) / f(2), np.asarray(a_par, f) / f(2)
+    else:
+        pmin, pmax = -np.asarray(n_row, f) / f(2), np.asarray(n_row, f) / f(2)
+    span = (pmax - pmin).astype(f)
+    with np.errstate(divide="ignore"):  # a plane normal across a cubic box gives tpev = +Inf, as in Fortran
+        tpe1, tpe2, tpev = (f(np.float64(twopi) / np.float64(f(np.dot(v, span)))) for v in (e1, e2, ev))
+    ainv = np.eye(3, dtype=f) if a_cell_inv is None else np.asarray(a_cell_inv, f)
+    bz_n = f(bz_n)
+    if nsuper > 1:
+        e1p = (twopi * (ainv @ e1).astype(np.float64)).astype(f)
+        e2p = (twopi * (ainv @ e2).astype(np.float64)).astype(f)
+        e1pn, e2pn = f(np.sqrt(np.dot(e1p, e1p))), f(np.sqrt(np.dot(e2p, e2p)))
+        cosa = f(np.dot(e1p, e2p) / (e1pn * e1pn))
+        bz_ny = max(bz_n, f(bz_n * (e1pn / (e2pn * f(np.sqrt(f(1) - cosa * cosa))))))
+        bz_nx = f(bz_n + bz_ny * abs(cosa)) if abs(cosa) > 0 else bz_n
+        n_qx, n_qy = int(twopi * float(bz_nx) / float(tpe1)), int(twopi * float(bz_ny) / float(tpe2))
+    else:
+        bz_nx = bz_ny = bz_n
+        n_qx, n_qy = int(f(bz_nx / tpe1)), int(f(bz_ny / tpe2))  # REAL(4) / REAL(4)
+    if n_qx > 1 and n_qx % 2:
+        n_qx += 1
+    if n_qy > 1 and n_qy % 2:
+        n_qy += 1
+    qc = np.asarray(q_center, f)
+    qx0, qy0 = f(np.dot(e1, qc) / n1), f(np.dot(e2, qc) / n2)
+    qx_min = qx_max = qx0
+    qy_min = qy_max = qy0
+    if n_qx > 1:
+        qx_min, qx_max = f(qx0 - f(0.5) * bz_nx), f(qx0 + f(0.5) * bz_nx)
+    if n_qy > 1:
+        qy_min, qy_max = f(qy0 - f(0.5) * bz_ny), f(qy0 + f(0.5) * bz_ny)
+    return dict(e1=e1, e2=e2, ev=ev, tpe=(tpe1, tpe2, tpev), bz_nx=bz_nx, bz_ny=bz_ny, n_qx=n_qx, n_qy=n_qy,
+                qx_range=(qx_min, qx_max), qy_range=(qy_min, qy_max), n_modes=(n_qx, n_qy),
+                engine_kw=dict(axes=np.stack([e1, e2, ev]).astype(np.float64),
+                               tpe=tuple(float(t) for t in (tpe1, tpe2, tpev)), q0=tuple(float(v) for v in qc)))

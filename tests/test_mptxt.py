@@ -106,3 +106,33 @@ def test_full_table_shape():
     assert all(len(r) == 2 * 51 + 1 + 2 * 10 for r in rows)  # two full groups, one of two values
     back = _parse_rows(rows)
     assert np.allclose(back, cs, rtol=5e-3)
+
+
+def test_qplane_reproduces_the_bzo_map_axes(tables):
+    """The Q-plane set-up (mp_sqom56.f90:800-857, 884-907) for the BZO run
+    (32^3 cells, 5 Brillouin zones, (hk0) plane, Q centre (2, 2, 0)) gives the
+    pixel counts and axes the reference's map files print."""
+    from mp_tools_amd import sqw
+    g = sqw.qplane((32, 32, 32), 5, (1, 0, 0), (0, 1, 0), (2, 2, 0))
+    assert (g["n_qx"], g["n_qy"]) == (160, 160)
+    assert mptxt.ld_record("Plot size X (lines):  ", g["n_qx"], mptxt._R4(g["qx_range"][0]),
+                           mptxt._R4(mptxt._grid_last(*g["qx_range"], g["n_qx"]))) == tables["07"]["header"][15]
+    assert mptxt.ld_record("Plot size Y (columns):", g["n_qy"], mptxt._R4(g["qy_range"][0]),
+                           mptxt._R4(mptxt._grid_last(*g["qy_range"], g["n_qy"]))) == tables["07"]["header"][16]
+    np.testing.assert_allclose(g["tpe"][:2], (2 * np.pi / 32, 2 * np.pi / 32), rtol=1e-7)
+
+
+def test_qplane_closed_forms():
+    from mp_tools_amd import sqw
+    # (hhl): e1 = [110]/sqrt2, e2 = [001]; e1 . (n_row box) = 32 sqrt2 cells
+    g = sqw.qplane((32, 32, 32), 4, (1, 1, 0), (0, 0, 1), (1, 1, 0))
+    np.testing.assert_allclose(g["e1"], np.array([1, 1, 0]) / np.sqrt(2), rtol=1e-7)
+    np.testing.assert_allclose(g["ev"], np.array([1, -1, 0]) / np.sqrt(2), rtol=1e-7)
+    n = int(4 * 32 * np.sqrt(2))
+    assert g["n_qx"] == n + (n % 2) and g["n_qy"] == 4 * 32
+    # qx centre: (e1 . q) / |e1_in| = (1 + 1) / sqrt2 / sqrt2 = 1
+    assert abs(float(g["qx_range"][0]) - (1.0 - 2.0)) < 1e-6 and abs(float(g["qy_range"][0]) + 2.0) < 1e-6
+    # one bulk box of 40 A (nsuper 1): pixels = bz_n / tpe, bz_n in 1/A
+    g = sqw.qplane((1, 1, 1), 2.0, q_center=(0, 0, 0), a_par=(40.0, 40.0, 40.0), nsuper=1)
+    assert g["n_qx"] == int(np.float32(2.0) / np.float32(2 * np.pi / 40.0)) + (int(np.float32(2.0) / np.float32(2 * np.pi / 40.0)) % 2)
+    assert g["engine_kw"]["axes"].shape == (3, 3)
