@@ -160,3 +160,37 @@ def test_sqw_end_to_end(gpu_lib, precision, eps, tol):
     Sr, Str = O.sqw_direct(F, wsc=w, n_int=T, dt=0.01, nat=nat, n_f=T // 2 + 1)
     assert rel_l2(St.cpu().numpy(), Str) < tol
     assert rel_l2(S.cpu().numpy(), Sr) < tol
+
+
+def test_map_file_end_to_end(gpu_lib, tmp_path):
+    """The reference's inputs -> Q-plane set-up -> engine total map -> map file
+    (mp_sqom56.f90:800-857, 1381-1387, 1574, 1994-2045): the written map parses
+    back to the oracle's S_tot to the file's three digits, rows along e1."""
+    import torch
+    from mp_tools_amd import mptxt
+    from mp_tools_amd.sqw import SqwEngine, qplane
+    ncell, T = 6, 4
+    pos, counts = _traj(ncell, T, seed=7)
+    g = qplane((ncell,) * 3, 2, (1, 0, 0), (0, 1, 0), q_center=(1, 1, 0))
+    nm = g["n_modes"]
+    assert nm == (12, 12)
+    w = np.array([0.507, 0.716, 0.580])
+    nat = float(np.sum(counts))
+    eng = SqwEngine(2, nm, counts, eps=1e-10, precision="fp64", weights=w, **g["engine_kw"])
+    amp = eng.amplitudes(torch.tensor(pos, device="cuda"))
+    S, St = eng.reduce(amp, n_int=T, dt=0.01, nat=nat, n_f=T // 2 + 1)
+    torch.cuda.synchronize()
+    F = _oracle_amps(pos, counts, 2, nm, g["engine_kw"]["q0"], w, axes=g["engine_kw"]["axes"],
+                     tpe=g["engine_kw"]["tpe"])
+    C = (F * w[None, :, None]).sum(axis=1)
+    st_ref = (np.abs(C) ** 2).sum(axis=0) / (T * nat * nat)
+    assert rel_l2(St.cpu().numpy(), st_ref) < 1e-8
+    cs = mptxt.cs_plot_total(St.cpu().numpy(), *nm)
+    path = str(tmp_path / "map.txt")
+    mptxt.write_sqom_table(path, cs, n_row=(ncell,) * 3, at_names=["A", "B", "O"], at_occup=[1, 1, 3],
+                           b_coh=w, q_center=(1, 1, 0), qx_range=g["qx_range"], qy_range=g["qy_range"])
+    lines = open(path).read().split("\n")
+    assert lines[15].split()[-3:] == ["12", "0.00000000", "1.83333337"]  # 12 pixels from Q = 0, 1/6 apart
+    back = np.array([[float(v) for v in r.split()] for r in lines[18:18 + nm[0]]])
+    ref = mptxt.cs_plot_total(st_ref, *nm)
+    np.testing.assert_allclose(back, ref, rtol=5e-3, atol=1e-6 * ref.max())
