@@ -192,6 +192,15 @@ def main():
     res["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                        "algorithmic_bytes_per_launch": per_launch_bytes}
+    # the spread kernel is bound by VALU issue, not HBM: its measured share of
+    # the SIMDs' VALU issue slots (SQ_INSTS_VALU x 4 cycles / (SIMDs x cycles))
+    cfile = os.path.join(ROOT, "profiles", "c2_fp32_instruction_counters.txt")
+    if dom == "spread_xfft" and os.path.exists(cfile):
+        for line in open(cfile):
+            if line.startswith("k_spread_narrow") and line.rstrip().endswith("%"):
+                res["roofline"]["valu_issue_frac"] = float(line.split()[-1].rstrip("%")) / 100.0
+                res["roofline"]["valu_issue_source"] = "profiles/c2_fp32_instruction_counters.txt"
+                break
     for k in stages:
         if k in alg:
             stages[k]["alg_GBps"] = alg[k] / (stages[k]["ms_per_step"] * 1e-3) / 1e9
