@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=4)
+    ap.add_argument("--cpu-frames", type=int, default=20)
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--workload", choices=("sqw", "batch", "serve", "sql"), default="sqw",
                     help="sqw: the S(Q,w) trajectory benchmark (default line); batch: 64 uneven jobs "
