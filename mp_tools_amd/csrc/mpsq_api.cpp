@@ -33,6 +33,15 @@ struct Err {
         if (e_ != hipSuccess) throw Err{MPSQ_ERR_DEVICE};                                                             \
     } while (0)
 
+// Error flags the kernels raise in a device int: bit 0 a non-finite point
+// coordinate, bit 1 an internal bound check of the spread (a candidate run
+// position outside the sorted records; never expected).
+static void throw_bad(int hbad)
+{
+    if (hbad & 2) throw Err{MPSQ_ERR_DEVICE};
+    if (hbad & 1) throw Err{MPSQ_ERR_NONFINITE_POINT};
+}
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -584,13 +593,16 @@ template <typename R> struct Core {
         const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
         return spread_lds<R>(by * bz, g.n1, g.w);
     }
-    SpreadJob<R> spreadjob(int ntr, const PtRec<R> *srec, const Cx<R> *cs, int64_t c_tstride, const int64_t *bstart,
-                           const int *bmax, int bin_tstride, Cx<R> *G1, const float *amax, int final_conj) const
+    SpreadJob<R> spreadjob(int ntr, const PtRec<R> *srec, int64_t nrec, int *err, const Cx<R> *cs, int64_t c_tstride,
+                           const int64_t *bstart, const int *bmax, int bin_tstride, Cx<R> *G1, const float *amax,
+                           int final_conj) const
     {
         SpreadJob<R> J{};
         J.g = g;
         J.rp1 = rp[0];
         J.srec = srec;
+        J.nrec = nrec;
+        J.err = err;
         J.cs = cs;
         J.c_tstride = c_tstride;
         J.binstart = bstart;
@@ -721,7 +733,7 @@ template <typename R> struct Plan {
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
-        if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
+        throw_bad(hbad);
         M = M_;
     }
 
@@ -763,7 +775,8 @@ template <typename R> struct Plan {
             am = amax.as<float>();
         }
         // one-entry job tables: spread, then the column passes
-        std::vector<SpreadJob<R>> sj(1, c.spreadjob(ntrans, srec.as<PtRec<R>>(), cs.as<Cx<R>>(), M,
+        launch_zero_i32(bad.as<int>(), 1, c.st);
+        std::vector<SpreadJob<R>> sj(1, c.spreadjob(ntrans, srec.as<PtRec<R>>(), M, bad.as<int>(), cs.as<Cx<R>>(), M,
                                                      binstart.as<int64_t>(),
                                                      sizeof(R) == 4 ? binmax.as<int>() : nullptr, 0, g1, am,
                                                      (c.dim == 1 && c.iflag > 0) ? 1 : 0));
@@ -793,8 +806,11 @@ template <typename R> struct Plan {
                                   (int)Core<R>::col_blocks(cz[0]), true, colfft_lds<R>(cz[0].cp, true), c.st);
         HIPCHK(hipGetLastError());
         if (!on_device) {
+            int hbad = 0;
+            HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
             HIPCHK(hipMemcpyAsync(f_out, fd, sizeof(Cx<R>) * (size_t)NQ * ntrans, hipMemcpyDeviceToHost, c.st));
             HIPCHK(hipStreamSynchronize(c.st));
+            throw_bad(hbad);
         }
     }
 
@@ -1045,7 +1061,7 @@ template <typename R> struct Engine {
         HIPCHK(hipMemcpyAsync(&hbad, ws.bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
         HIPCHK(hipStreamSynchronize(c.st));
         tm.collect();
-        if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
+        throw_bad(hbad);
     }
 
     void reduce(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot)
@@ -1077,7 +1093,8 @@ template <typename R> struct Engine {
         p.norm_tot = 1.0 / ((double)T * r->nat * r->nat);
         const Cx<R> *amp = static_cast<const Cx<R> *>(d_amp);
         RadixPlan rp{};
-        const bool single = (p.n_frame == 1) && (T <= sqw_tile_elems<R>()) && radix_plan((int)T, rp);
+        const bool single = (p.n_frame == 1) && (T <= sqw_tile_elems<R>()) &&
+                            sqw_single_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
         if (single) {
             if (win_n != (int)T || win_kind != r->win_kind) {
                 win_h.resize((size_t)T);
@@ -1097,7 +1114,9 @@ template <typename R> struct Engine {
                 rpT = rp;
             }
             int cols = 1;
-            while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS) cols *= 2;
+            while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS &&
+                   sqw_single_lds<R>(T, 2 * cols) <= SQW_MAX_LDS)
+                cols *= 2;
             const int ev = tm.begin(ST_REDUCE, st);
             launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, st);
             tm.end(ev, st);
@@ -1268,7 +1287,8 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             const Core<R> &c = e.core;
             if (sclass(c) != widths[wi]) continue;
             const int ntr = parts[q].b * tpf(q);
-            SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), ws.cs.template as<Cx<R>>(), 0,
+            SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), npts, ws.bad.template as<int>(),
+                                         ws.cs.template as<Cx<R>>(), 0,
                                          ws.binstart.template as<int64_t>() + bso[q],
                                          sizeof(R) == 4 ? ws.binmax.template as<int>() + bso[q] : nullptr, 1,
                                          ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
@@ -1388,7 +1408,9 @@ template <typename R> struct Batch {
     // captured sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    std::vector<const void *> gkey;  // pointers / sizes the graph was captured with
+    // pointers, sizes and every reduction argument the graph was captured
+    // with (the reductions' parameters are baked into the captured launches)
+    std::vector<uint64_t> gkey;
 
     ~Batch()
     {
@@ -1473,7 +1495,7 @@ template <typename R> struct Batch {
         int hbad = 0;
         HIPCHK(hipMemcpyAsync(&hbad, ws.bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        if (hbad) throw Err{MPSQ_ERR_NONFINITE_POINT};
+        throw_bad(hbad);
     }
 
     // use_graph: the first call with a given set of pointers runs eagerly
@@ -1483,18 +1505,29 @@ template <typename R> struct Batch {
              double *const *d_S, double *const *d_Stot, bool use_graph)
     {
         HIPCHK(hipSetDevice(device));
-        std::vector<const void *> key;
+        std::vector<uint64_t> key;
+        auto kp = [&](const void *p) { key.push_back((uint64_t)(uintptr_t)p); };
+        auto kd = [&](double v) {
+            uint64_t b;
+            std::memcpy(&b, &v, sizeof b);
+            key.push_back(b);
+        };
+        key.push_back(r ? 1u : 0u);
         for (size_t j = 0; j < jobs.size(); ++j) {
-            key.push_back(d_pos[j]);
-            key.push_back((const void *)(intptr_t)T[j]);
-            key.push_back(d_amp[j]);
-            key.push_back(d_S ? d_S[j] : nullptr);
-            key.push_back(d_Stot ? d_Stot[j] : nullptr);
+            kp(d_pos[j]);
+            key.push_back((uint64_t)T[j]);
+            kp(d_amp[j]);
+            kp(d_S ? d_S[j] : nullptr);
+            kp(d_Stot ? d_Stot[j] : nullptr);
             if (r) {
-                key.push_back((const void *)(intptr_t)r[j].n_int);
-                key.push_back((const void *)(intptr_t)r[j].n_f);
-                key.push_back((const void *)(intptr_t)r[j].m_lo);
-                key.push_back(r[j].d_qfac);
+                key.push_back((uint64_t)r[j].n_int);
+                key.push_back((uint64_t)r[j].win_kind);
+                kd(r[j].dt);
+                kd(r[j].nat);
+                key.push_back((uint64_t)r[j].m_lo);
+                key.push_back((uint64_t)r[j].n_f);
+                key.push_back((uint64_t)r[j].subtract_mean);
+                kp(r[j].d_qfac);
             }
         }
         if (use_graph && gexec && key == gkey) {

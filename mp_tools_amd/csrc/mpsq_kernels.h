@@ -109,6 +109,8 @@ template <typename R> struct SpreadJob {
     Geo g;
     RadixPlan rp1;
     const PtRec<R> *srec;                // sorted records (absolute positions)
+    int64_t nrec;                        // records in srec (bound of every run position)
+    int *err;                            // error flags of the run (bit 1: a run position out of range)
     const typename cplx<R>::T *cs;       // sorted strengths, transform t at + t*c_tstride
     int64_t c_tstride;
     const int64_t *binstart;             // transform t's block at + t*bin_tstride*(nbins+1)
@@ -228,6 +230,10 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
 // X-ray form-factor weights w[s][Q] on the mode grid N (see sqw_kernels.hip)
 void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double *coef, float *w, hipStream_t st);
 template <typename R> int sqw_tile_elems();
+// dynamic LDS of a k_sqw_single workgroup over T frames and cols Q columns;
+// launches keep it <= SQW_MAX_LDS (the kernel's own static LDS is 1 KB)
+template <typename R> size_t sqw_single_lds(int64_t T, int cols);
+constexpr size_t SQW_MAX_LDS = 150 * 1024;
 template <typename R>
 void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,
                        const typename cplx<R>::T *tw, const double *win, const float *qfac, double *S, double *Stot,

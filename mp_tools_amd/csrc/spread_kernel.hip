@@ -51,6 +51,7 @@
 #include <mutex>
 
 #include "mpsq_kernels.h"
+#include "spread_tiles.h"
 
 namespace mpsq {
 
@@ -68,18 +69,10 @@ struct NarrowCfg {
 };
 constexpr int MAX_WIDE_PASSES = 4;  // narrow tile rows / wide pass rows
 
-constexpr int MAX_RUNS = 64;     // candidate runs of one tile (two per visited z-bin row)
 constexpr int MAX_TILE_B = 16;   // by, bz <= 16
 constexpr int NARROW_MAX = 256;  // largest covering bound of a 32-bit (narrow) FP32 tile
 // counters after the 8 per-XCD tile counters: fallback items queued, taken
 constexpr int CTR_FB_COUNT = 8 * XCD_CTR_STRIDE, CTR_FB_TAKE = 9 * XCD_CTR_STRIDE;
-
-__device__ __forceinline__ int floordiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
-__device__ __forceinline__ int posmod(int a, int n)
-{
-    const int r = a % n;
-    return r < 0 ? r + n : r;
-}
 
 // phi(t) = exp(beta (sqrt(1 - ic t^2) - 1)), |t| <= w/2.  FP32: hardware
 // exp2 / sqrt (relative error ~1e-6 only where phi < 1e-6, i.e. absolute
@@ -282,39 +275,6 @@ __device__ __forceinline__ void tile_decode(const SpreadJob<R> &J, int id, int &
     z0 = tz * bz;
 }
 
-// sort bins a tile visits: y bins [ylo, ylo + nyb), z bins [zlo, zlo + nzb) (cyclic)
-__device__ __forceinline__ void tile_bins(const Geo &g, int y0, int z0, int &ylo, int &nyb, int &zlo, int &nzb)
-{
-    const int w = g.w, by = (g.dim >= 2) ? g.by : 1, bz = (g.dim >= 3) ? g.bz : 1;
-    ylo = 0;
-    nyb = 1;
-    zlo = 0;
-    nzb = 1;
-    if (g.dim >= 2) {
-        ylo = floordiv(y0 - (w - 1), g.BY);
-        nyb = floordiv(y0 + by - 1, g.BY) - ylo + 1;
-        if (nyb >= g.nbY) {
-            ylo = 0;
-            nyb = g.nbY;
-        }
-    }
-    if (g.dim >= 3) {
-        zlo = floordiv(z0 - (w - 1), g.BZ);
-        nzb = floordiv(z0 + bz - 1, g.BZ) - zlo + 1;
-        if (nzb >= g.nbZ) {
-            zlo = 0;
-            nzb = g.nbZ;
-        }
-    }
-}
-
-__device__ __forceinline__ int nruns_of(const Geo &g, int y0, int z0)
-{
-    int ylo, nyb, zlo, nzb;
-    tile_bins(g, y0, z0, ylo, nyb, zlo, nzb);
-    return 2 * nzb;
-}
-
 // candidate runs, strength bound and covering bound of the current and the
 // next tile (double-buffered), and the next tile's id, in LDS
 struct RunTable {
@@ -338,37 +298,8 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
     const int nruns = 2 * nzb;  // <= MAX_RUNS (checked by the host)
     int64_t b0 = 0;
     int cnt = 0;
-    if (lane < nruns) {
-        const int64_t *bs = J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1);
-        const int k = lane >> 1, part = lane & 1;
-        const int zb = posmod(zlo + k, g.nbZ);
-        int ya = 0, yb = 0;  // bin range [ya, yb) of this part
-        const int a = ylo, b = ylo + nyb;
-        if (nyb >= g.nbY) {
-            if (part == 0) yb = g.nbY;
-        } else if (a < 0) {
-            if (part == 0) {
-                ya = a + g.nbY;
-                yb = g.nbY;
-            } else {
-                yb = b;
-            }
-        } else if (b > g.nbY) {
-            if (part == 0) {
-                ya = a;
-                yb = g.nbY;
-            } else {
-                yb = b - g.nbY;
-            }
-        } else if (part == 0) {
-            ya = a;
-            yb = b;
-        }
-        if (yb > ya) {
-            b0 = bs[zb * g.nbY + ya];
-            cnt = (int)(bs[zb * g.nbY + yb] - b0);
-        }
-    }
+    if (lane < nruns)
+        tile_run(g, J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1), ylo, nyb, zlo, lane, b0, cnt);
     int cb = 0x7fffffff;
     if (bound && J.binmax) {
         const int *bm = J.binmax + (int64_t)t * J.bin_tstride * (g.nbins + 1);
@@ -402,6 +333,11 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
 // one lane's candidate (sorted record and strength): candidate v of the
 // tile whose runs are in buffer b; e is the lane's run cursor (v ascending
 // between resets)
+// The run table is published before it is read (barriers), so rpre is the
+// nondecreasing prefix of the tile's run sizes with rpre[nruns] = total and
+// the walk stops at e < nruns; the bounds below only guard against a table
+// that breaks this (the walk stops at the table's end, the load is skipped
+// and the launch's error flag is raised instead of reading out of range).
 template <typename R> struct Cand {
     PtRec<R> pr;
     typename cplx<R>::T cv;
@@ -411,8 +347,13 @@ template <typename R> struct Cand {
     {
         have = v < total;
         if (have) {
-            while (S.rpre[b][e + 1] <= v) ++e;
+            while (e < MAX_RUNS - 1 && S.rpre[b][e + 1] <= v) ++e;
             const int64_t p = S.rbeg[b][e] + (v - S.rpre[b][e]);
+            if ((uint64_t)p >= (uint64_t)J.nrec) {
+                have = false;
+                atomicOr(J.err, 2);
+                return;
+            }
             pr = J.srec[p];
             cv = J.cs[(int64_t)tt * J.c_tstride + p];
         }

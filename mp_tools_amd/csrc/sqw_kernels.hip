@@ -245,6 +245,12 @@ template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };
 
 template <typename R> int sqw_tile_elems() { return SqwTiles<R>::E * SqwTiles<R>::NT; }
 
+template <typename R> size_t sqw_single_lds(int64_t T, int cols)
+{
+    return (size_t)T * (size_t)(cols + 1) * sizeof(typename cplx<R>::T) +
+           sizeof(double) * 3 * (SqwTiles<R>::NT / 64) * (size_t)cols;
+}
+
 template <typename R>
 void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,
                        const typename cplx<R>::T *tw, const double *win, const float *qfac, double *S, double *Stot,
@@ -252,11 +258,11 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 {
     constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
     const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
-    const size_t lds = (size_t)p.T * (cols + 1) * sizeof(typename cplx<R>::T) + sizeof(double) * 3 * (NT / 64) * cols;
+    const size_t lds = sqw_single_lds<R>(p.T, cols);  // <= SQW_MAX_LDS (the caller's choice of cols)
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqw_single<R, E, NT>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
         attr = true;
     }
     hipLaunchKernelGGL((k_sqw_single<R, E, NT>), dim3(nb), dim3(NT), lds, st, p, rp, cols, make_fdiv((uint32_t)cols),
@@ -275,6 +281,7 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
 
 #define MPSQ_SQW_INST(R)                                                                                              \
     template int sqw_tile_elems<R>();                                                                                 \
+    template size_t sqw_single_lds<R>(int64_t, int);                                                                  \
     template void launch_sqw_single<R>(const SqwParams &, const RadixPlan &, int, const cplx<R>::T *,                \
                                        const cplx<R>::T *, const double *, const float *, double *, double *,        \
                                        hipStream_t);                                                                  \

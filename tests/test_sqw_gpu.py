@@ -110,6 +110,11 @@ RED_CASES = [
     (12, 1, 0, 0, 1, True, False),       # n_int = 1: window 1
     (512, 512, 0, 0, 257, True, False),   # one window over 512 frames (the C2 shape)
     (256, 256, 1, -100, 200, False, True),  # 256 frames, negative m_lo, per-Q factor
+    # long single windows: the LDS time FFT at its largest sizes (FP64 2048 / 4096,
+    # FP32 up to 8192 frames; FP64 8192 takes the sliding path)
+    (2048, 2048, 0, 0, 64, True, False),
+    (4096, 4096, 1, -5, 40, True, True),
+    (8192, 8192, 0, 0, 24, True, False),
 ]
 
 
