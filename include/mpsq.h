@@ -178,10 +178,52 @@ typedef struct mpsq_reduce_spec {
     const float *d_qfac;      /* device [NQ] factor on C' (q2_norm when j_qsq != 1), or NULL */
 } mpsq_reduce_spec;
 
+/* The reference's per-Q tables of a Q plane (mp_sqom56.f90:909-936),
+ * computed on the device in its REAL(4) arithmetic: for pixel (i, j),
+ * i = 0..n_qx-1 fastest (the CMCL order of the plane's amplitudes, k1 =
+ * i - n_qx/2), q1 = (qx_min + i * (bz_nx / n_qx)) e1, q2 = (qy_min + j *
+ * (bz_ny / n_qy)) e2 and
+ *   q2_norm = 1 / sqrt(|q1 + q2|^2 + |q_v|^2)  (1 where that is below 1e-8),
+ *             the factor on C' when S(Q)/Q^2 scaling is off (j_qsq != 1);
+ *   x_ffq[s] = c + sum_k a_k exp(-0.25 b_k q_sq'),  q_sq' = 0.25 (|q1' + q2'|^2
+ *             + |q_vp|^2) with the same pixel coefficients on e1p, e2p (1/A),
+ *             the X-ray weights of species s (j_weight = 3).
+ * x_ffpar: host [nsp][9] coefficients as the reference reads them
+ * (a1, b1, a2, b2, a3, b3, a4, b4, c; mp_sqom56.f90:495); nsp <= 64.
+ * d_q2_norm: device float[n_qx * n_qy] or NULL; d_x_ffq: device
+ * float[nsp][n_qx * n_qy] or NULL.  Enqueued on stream (NULL: the default
+ * stream). */
+typedef struct mpsq_qplane {
+    int32_t n_qx, n_qy;
+    float qx_min, qy_min;      /* dot(e1, q_center)/|e1| - bz_nx/2 (and y), :895-906 */
+    float bz_nx, bz_ny;        /* Brillouin-zone extents of the plane */
+    float e1[3], e2[3];        /* normalised plane vectors (r.l.u.) */
+    float q_v[3];              /* (dot(ev, q_center)/|ev|) ev (:911) */
+    float e1p[3], e2p[3];      /* plane vectors in 1/A (:812-820) */
+    float q_vp[3];             /* (dot(evp, q_center)/|evp|) evp (:920) */
+} mpsq_qplane;
+int mpsq_qtables(const mpsq_qplane *qp, int nsp, const float *x_ffpar, float *d_q2_norm, float *d_x_ffq,
+                 void *stream);
+
 /* d_S: device double[n_f][NQ]; d_Stot: device double[NQ] or NULL. */
 int mpsq_sqw_reduce(mpsq_sqw eng, const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S,
                     double *d_Stot);
 int mpsq_sqw_destroy(mpsq_sqw eng);
+/* Inspection of the engine's staged, bin-sorted points (a1-a3, a4.1), for
+ * bit-exact parity checks of the selection, fold and sort: stages and sorts
+ * frame `frame` of d_pos alone and writes, per sorted position k < n (n must
+ * equal the active atoms per frame; transforms in order, one per frame when
+ * the species are merged into one transform, else one per active species):
+ * atom[k] = the atom's index in the frame (mp_sqom56.f90:1104-1137 keeps the
+ * species-major order; the sort is stable), cell[3k..3k+2] = its first
+ * kernel cell on the fine grid, keep[k] = 1 if it passed the vacancy and
+ * window selection (mp_sqom56.f90:1106-1113,1123-1132), else 0 (its strength
+ * is then 0).  Host arrays; any may be NULL.  Synchronises. */
+int mpsq_sqw_debug_sorted(mpsq_sqw eng, const float *d_pos, int64_t frame, int32_t *atom, int32_t *cell,
+                          int32_t *keep, int64_t n);
+/* Fine grid, kernel width, spread tile and sort-bin shape of an engine, in
+ * the layout of mpsq_plan_info. */
+int mpsq_sqw_info(mpsq_sqw eng, int64_t *out);
 /* Synchronise the engine's stream. */
 int mpsq_sqw_sync(mpsq_sqw eng);
 /* Per-stage timing with HIP events recorded on the engine's stream around

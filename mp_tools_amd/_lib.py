@@ -35,7 +35,7 @@ EXPORTS = [
     "mpsq_makeplan_", "mpsq_setpts_", "mpsq_execute_", "mpsq_destroy_",
     "mpsq_sqw_default_spec", "mpsq_sqw_create", "mpsq_sqw_amplitudes", "mpsq_sqw_reduce",
     "mpsq_sqw_destroy", "mpsq_sqw_sync", "mpsq_sqw_nq", "mpsq_sqw_nact", "mpsq_version",
-    "mpsq_sqw_set_timing", "mpsq_sqw_get_timing",
+    "mpsq_sqw_set_timing", "mpsq_sqw_get_timing", "mpsq_sqw_debug_sorted", "mpsq_sqw_info", "mpsq_qtables",
     "mpsq_batch_create", "mpsq_batch_run", "mpsq_batch_set_budget", "mpsq_batch_set_timing",
     "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
     "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
@@ -51,6 +51,12 @@ class MpsqError(RuntimeError):
                  5: "HIP device/runtime failure", 6: "execute before setpts"}
         super().__init__(f"{where}: mpsq error {code} ({names.get(code, 'unknown')})")
         self.code = code
+
+
+class QPlane(C.Structure):
+    _fields_ = [("n_qx", C.c_int32), ("n_qy", C.c_int32), ("qx_min", C.c_float), ("qy_min", C.c_float),
+                ("bz_nx", C.c_float), ("bz_ny", C.c_float), ("e1", C.c_float * 3), ("e2", C.c_float * 3),
+                ("q_v", C.c_float * 3), ("e1p", C.c_float * 3), ("e2p", C.c_float * 3), ("q_vp", C.c_float * 3)]
 
 
 class Opts(C.Structure):
@@ -143,6 +149,9 @@ def load():
     L.mpsq_sqw_nact.argtypes = [vp]
     L.mpsq_version.restype = C.c_char_p
     L.mpsq_sqw_set_timing.argtypes = [vp, i32]
+    L.mpsq_sqw_debug_sorted.argtypes = [vp, vp, i64, vp, vp, vp, i64]
+    L.mpsq_qtables.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.mpsq_sqw_info.argtypes = [vp, vp]
     L.mpsq_sqw_get_timing.argtypes = [vp, vp, vp, i32]
     L.mpsq_batch_create.argtypes = [i32, vp, vp]
     L.mpsq_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, i32]

@@ -391,8 +391,9 @@ template <typename R> struct Core {
     int64_t N[3] = {1, 1, 1}, n[3] = {1, 1, 1};
     Geo g{};
     RadixPlan rp[3]{};
-    DevBuf tw[3], phinv[3];
+    DevBuf tw[3], tw64[3], phinv[3];
     std::vector<Cx<R>> tw_host[3];
+    std::vector<double2> tw64_host[3];
     std::vector<R> phinv_host[3];
     int device = 0;
     hipStream_t st = nullptr;
@@ -510,6 +511,7 @@ template <typename R> struct Core {
         for (int d = 0; d < dim; ++d) {
             if (!radix_plan((int)n[d], rp[d])) throw Err{MPSQ_ERR_SIZE};
             upload_twiddles<R>(tw[d], (int)n[d], st, tw_host[d]);
+            upload_twiddles<double>(tw64[d], (int)n[d], st, tw64_host[d]);
             const std::vector<double> f = deconv_factors(N[d], n[d], w, beta);
             phinv_host[d].assign(f.begin(), f.end());
             phinv[d].ensure(sizeof(R) * f.size());
@@ -566,6 +568,7 @@ template <typename R> struct Core {
         J.in = in;
         J.out = out;
         J.tw = tw[axis].template as<Cx<R>>();
+        J.tw64 = tw64[axis].template as<double2>();
         J.phinv = phinv[axis].template as<R>();
         J.wsc = wsc;
         J.wtab = final ? wtab : nullptr;
@@ -625,7 +628,7 @@ template <typename R> struct Core {
         J.tmajor = 0;
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
-        J.lds_tw = (int)(spread_lds_size() - (size_t)g.n1 * sizeof(Cx<R>));
+        J.lds_tw = (int)spread_lds_twiddles<R>(by * bz, g.n1, g.w);
         return J;
     }
     // estimated cost of one tile of this geometry with M points per transform
@@ -1062,6 +1065,43 @@ template <typename R> struct Engine {
         HIPCHK(hipStreamSynchronize(c.st));
         tm.collect();
         throw_bad(hbad);
+    }
+
+    // Stage and bin-sort frame f alone (the spread and column passes run too,
+    // into a scratch amplitude buffer) and copy the sorted records back:
+    // atom index, first kernel cell and selection flag per sorted position.
+    void debug_sorted(const float *d_pos, int64_t f, int32_t *atom, int32_t *cell, int32_t *keep, int64_t n)
+    {
+        Core<R> &c = core;
+        if (!d_pos || f < 0 || n != Mact) throw Err{MPSQ_ERR_BAD_ARG};
+        HIPCHK(hipSetDevice(c.device));
+        DevBuf amp;
+        amp.ensure(sizeof(Cx<R>) * (size_t)amp_frame_elems());
+        ws.bad.ensure(sizeof(int));
+        HIPCHK(hipMemsetAsync(ws.bad.p, 0, sizeof(int), c.st));
+        std::vector<Part<R>> parts(1);
+        parts[0].e = this;
+        parts[0].f0 = f;
+        parts[0].b = 1;
+        parts[0].pos = d_pos + (size_t)f * n_tot * 4;
+        parts[0].amp = amp.p;
+        run_parts<R>(parts, ws, ws.tab, c.st, nullptr);
+        std::vector<PtRec<R>> h((size_t)std::max<int64_t>(n, 1));
+        if (n > 0)
+            HIPCHK(hipMemcpyAsync(h.data(), ws.srec.p, sizeof(PtRec<R>) * (size_t)n, hipMemcpyDeviceToHost, c.st));
+        int hbad = 0;
+        HIPCHK(hipMemcpyAsync(&hbad, ws.bad.p, sizeof(int), hipMemcpyDeviceToHost, c.st));
+        HIPCHK(hipStreamSynchronize(c.st));
+        throw_bad(hbad);
+        for (int64_t k = 0; k < n; ++k) {
+            if (atom) atom[k] = h[k].idx;
+            if (cell) {
+                cell[3 * k] = h[k].lx;
+                cell[3 * k + 1] = h[k].ly;
+                cell[3 * k + 2] = h[k].lz;
+            }
+            if (keep) keep[k] = h[k].pad != (R)0 ? 1 : 0;
+        }
     }
 
     void reduce(const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S, double *d_Stot)
@@ -2229,6 +2269,59 @@ int mpsq_sqw_reduce(mpsq_sqw e, const void *d_amp, int64_t T, const mpsq_reduce_
         return MPSQ_ERR_DEVICE;
     }
     return 0;
+}
+
+int mpsq_sqw_debug_sorted(mpsq_sqw e, const float *d_pos, int64_t frame, int32_t *atom, int32_t *cell,
+                          int32_t *keep, int64_t n)
+{
+    if (!e) return MPSQ_ERR_BAD_ARG;
+    try {
+        if (e->ed) e->ed->debug_sorted(d_pos, frame, atom, cell, keep, n);
+        else e->ef->debug_sorted(d_pos, frame, atom, cell, keep, n);
+    } catch (const Err &er) {
+        return er.code;
+    } catch (...) {
+        return MPSQ_ERR_DEVICE;
+    }
+    return 0;
+}
+
+int mpsq_sqw_info(mpsq_sqw e, int64_t *out)
+{
+    if (!e || !out) return MPSQ_ERR_BAD_ARG;
+    const Geo &g = e->ed ? e->ed->core.g : e->ef->core.g;
+    const int64_t v[8] = {g.n1, g.n2, g.n3, g.w, g.by, g.bz, g.BY, g.BZ};
+    for (int k = 0; k < 8; ++k) out[k] = v[k];
+    return 0;
+}
+
+int mpsq_qtables(const mpsq_qplane *qp, int nsp, const float *x_ffpar, float *d_q2_norm, float *d_x_ffq,
+                 void *stream)
+{
+    if (!qp || qp->n_qx < 1 || qp->n_qy < 1 || (d_x_ffq && (nsp < 1 || nsp > 64 || !x_ffpar)))
+        return MPSQ_ERR_BAD_ARG;
+    QTableParams p{};
+    p.n_qx = qp->n_qx;
+    p.n_qy = qp->n_qy;
+    p.nsp = d_x_ffq ? nsp : 0;
+    p.qx_min = qp->qx_min;
+    p.qy_min = qp->qy_min;
+    p.dqx = qp->bz_nx / (float)qp->n_qx;
+    p.dqy = qp->bz_ny / (float)qp->n_qy;
+    for (int d = 0; d < 3; ++d) {
+        p.e1[d] = qp->e1[d];
+        p.e2[d] = qp->e2[d];
+        p.q_v[d] = qp->q_v[d];
+        p.e1p[d] = qp->e1p[d];
+        p.e2p[d] = qp->e2p[d];
+        p.q_vp[d] = qp->q_vp[d];
+    }
+    for (int s = 0; s < p.nsp; ++s)
+        for (int k = 0; k < 9; ++k) p.ff[s][k] = x_ffpar[9 * s + k];
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPSQ_ERR_DEVICE;
+    launch_qtables(p, d_q2_norm, d_x_ffq, (hipStream_t)stream);
+    return hipGetLastError() == hipSuccess ? 0 : MPSQ_ERR_DEVICE;
 }
 
 int mpsq_sqw_sync(mpsq_sqw e)

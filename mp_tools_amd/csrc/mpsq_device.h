@@ -43,6 +43,47 @@ template <typename C> __device__ __forceinline__ C cmul_mi(C a) { return cmk<C>(
 template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.x, -a.y); }
 
 // ---------------------------------------------------------------------------
+// Loads and stores through global-address-space pointers.  A pointer read
+// from a job table is generic, and generic accesses compile to flat
+// instructions, which count against the LDS counter too: a wait for an LDS
+// result then also waits for every flat load in flight (a candidate
+// prefetch, say).  These helpers issue global_load / global_store.
+// ---------------------------------------------------------------------------
+#define MPSQ_GLOBAL __attribute__((opencl_global))
+typedef unsigned int mpsq_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int mpsq_u32x4 __attribute__((ext_vector_type(4)));
+template <int N> struct U32x4s {
+    mpsq_u32x4 v[N];
+};
+template <typename T> __device__ __forceinline__ T gload(const T *p)
+{
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, *(const MPSQ_GLOBAL unsigned int *)p);
+    } else if constexpr (sizeof(T) == 8) {
+        return __builtin_bit_cast(T, *(const MPSQ_GLOBAL mpsq_u32x2 *)p);
+    } else {
+        static_assert(sizeof(T) % 16 == 0, "gload: 4, 8 or 16n bytes");
+        U32x4s<sizeof(T) / 16> b;
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(T) / 16); ++k) b.v[k] = ((const MPSQ_GLOBAL mpsq_u32x4 *)p)[k];
+        return __builtin_bit_cast(T, b);
+    }
+}
+template <typename T> __device__ __forceinline__ void gstore(T *p, const T &v)
+{
+    if constexpr (sizeof(T) == 4) {
+        *(MPSQ_GLOBAL unsigned int *)p = __builtin_bit_cast(unsigned int, v);
+    } else if constexpr (sizeof(T) == 8) {
+        *(MPSQ_GLOBAL mpsq_u32x2 *)p = __builtin_bit_cast(mpsq_u32x2, v);
+    } else {
+        static_assert(sizeof(T) % 16 == 0, "gstore: 4, 8 or 16n bytes");
+        const U32x4s<sizeof(T) / 16> b = __builtin_bit_cast(U32x4s<sizeof(T) / 16>, v);
+#pragma unroll
+        for (int k = 0; k < (int)(sizeof(T) / 16); ++k) ((MPSQ_GLOBAL mpsq_u32x4 *)p)[k] = b.v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // forward DFT butterflies, X_k = sum_n a_n exp(-2 pi i n k / R)
 // ---------------------------------------------------------------------------
 template <int R, typename C> struct Bfly;

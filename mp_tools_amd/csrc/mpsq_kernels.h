@@ -136,6 +136,7 @@ template <typename R> struct ColJob {
     const typename cplx<R>::T *in;
     typename cplx<R>::T *out;
     const typename cplx<R>::T *tw;
+    const double2 *tw64;                 // FP64 twiddles (passes that compute in FP64 on FP32 data)
     const R *phinv, *wsc;
     const float *wtab;
     int ncb, ngroups, blk0;
@@ -186,6 +187,8 @@ int spread_max_wide_passes();
 template <typename R> int spread_max_rows(int n1);
 template <typename R> int spread_group_rows(int rows, int n1);
 template <typename R> size_t spread_lds(int rows, int n1, int w);
+// byte offset of the twiddles in a spread workgroup's dynamic LDS
+template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w);
 // binmax[bs0 + b] = largest number of bin b's points whose first kernel cell
 // lies in 3 cyclically consecutive x segments (a bound on how many of them
 // cover any one cell of a row)
@@ -229,6 +232,15 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
 // --- launch helpers (sqw_kernels.hip) ---
 // X-ray form-factor weights w[s][Q] on the mode grid N (see sqw_kernels.hip)
 void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double *coef, float *w, hipStream_t st);
+// mp_sqom's Q tables of a Q plane (q2_norm, x_ffq; see sqw_kernels.hip)
+struct QTableParams {
+    int n_qx, n_qy, nsp;
+    float qx_min, qy_min, dqx, dqy;  // dqx = bz_nx / n_qx (REAL(4) division, as the reference)
+    float e1[3], e2[3], q_v[3];
+    float e1p[3], e2p[3], q_vp[3];
+    float ff[64][9];                 // a1, b1, a2, b2, a3, b3, a4, b4, c per species
+};
+void launch_qtables(const QTableParams &p, float *q2_norm, float *x_ffq, hipStream_t st);
 template <typename R> int sqw_tile_elems();
 // dynamic LDS of a k_sqw_single workgroup over T frames and cols Q columns;
 // launches keep it <= SQW_MAX_LDS (the kernel's own static LDS is 1 KB)

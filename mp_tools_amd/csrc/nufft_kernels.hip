@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void k_stage_traj(const StageJob *__restrict__
     rr.dx = (R)d[0];
     rr.dy = (R)d[1];
     rr.dz = (R)d[2];
-    rr.pad = (R)0;
+    rr.pad = keep ? (R)1 : (R)0;  // selection flag (read by the engine's debug view only)
     rec[p] = rr;
     key[p] = bin_key<R>(g, l[1], l[2]);
     C cv;
@@ -456,11 +456,15 @@ __global__ void k_absmax(int64_t M, const float2 *__restrict__ c, unsigned *__re
 // nsp transforms of group blockIdx.y with weights and writes [g][N][inner]
 // (conjugated for iflag > 0).
 // ---------------------------------------------------------------------------
-template <typename R, int E, int NT, bool FINAL>
+// RC is the arithmetic (and LDS tile) precision; the data in HBM stays
+// cplx<R> (RC = double on FP32 data runs the pass in FP64 with FP64
+// twiddles).
+template <typename R, typename RC, int E, int NT, bool FINAL>
 __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ jobs, const int *__restrict__ blk0s,
                                                int njobs)
 {
-    using C = typename cplx<R>::T;
+    using C = typename cplx<RC>::T;
+    using CS = typename cplx<R>::T;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const ColJob<R> &J = jobs[find_job(blk0s, njobs, blockIdx.x)];
     const ColPass &cp = J.cp;
@@ -468,7 +472,10 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
     const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
     C *tw_l = tile + n * cols;  // twiddles in LDS
     C *acc = tw_l + n;          // FINAL: species sum of the kept modes, [N][cols]
-    for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = J.tw[i];
+    const C *twg;
+    if constexpr (sizeof(RC) == sizeof(R)) twg = reinterpret_cast<const C *>(J.tw);
+    else twg = reinterpret_cast<const C *>(J.tw64);
+    for (int i = threadIdx.x; i < n; i += NT) tw_l[i] = twg[i];
     const int ncb = J.ncb;
     const int local = blockIdx.x - J.blk0;
     const int grp = local / (ncb * cp.outer);
@@ -489,13 +496,14 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
             C v[E];
             for (int u = 0; u < nspl; ++u) {
                 const int tu = FINAL ? t : grp * nspl + u;
-                const C *src = J.in + (int64_t)tu * cp.in_tstride + (int64_t)o * n * inner + col0;
-                const R wu = (!FINAL && J.wsc) ? J.wsc[u] : (R)1;
+                const CS *src = J.in + (int64_t)tu * cp.in_tstride + (int64_t)o * n * inner + col0;
+                const RC wu = (!FINAL && J.wsc) ? (RC)J.wsc[u] : (RC)1;
 #pragma unroll
                 for (int k = 0; k < E; ++k) {
                     const int i = threadIdx.x + k * NT;
                     const int r = (int)fdiv((uint32_t)i, cp.dcols), c = i - r * cols;
-                    const C x = (i < n * cols && c < ncol) ? src[r * inner + c] : cmk<C>((R)0, (R)0);
+                    const CS xs = (i < n * cols && c < ncol) ? src[r * inner + c] : cmk<CS>((R)0, (R)0);
+                    const C x = cmk<C>((RC)xs.x, (RC)xs.y);
                     v[k] = (u == 0) ? cscale(x, wu) : cadd(v[k], cscale(x, wu));
                 }
             }
@@ -507,14 +515,15 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
         }
         __syncthreads();
         lds_fft<C, E>(tile, J.rp, cols, cp.dcols, cols, 1, tw_l, true);
-        if (!FINAL) {
-            C *dst = J.out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
+        if constexpr (!FINAL) {
+            CS *dst = J.out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
             for (int i = threadIdx.x; i < N * cols; i += NT) {
                 const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 if (c >= ncol) continue;
                 int f = kk - h;
                 if (f < 0) f += n;
-                dst[(int64_t)kk * inner + c] = cscale(tile[f * cols + c], J.phinv[kk]);
+                const C x = cscale(tile[f * cols + c], (RC)J.phinv[kk]);
+                dst[(int64_t)kk * inner + c] = cmk<CS>((R)x.x, (R)x.y);
             }
         } else {
             for (int i = threadIdx.x; i < N * cols; i += NT) {
@@ -530,7 +539,7 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
         }
         __syncthreads();
     }
-    if (FINAL) {
+    if constexpr (FINAL) {
         C *dst = J.out + (int64_t)grp * cp.out_tstride + (int64_t)o * N * inner + col0;
         for (int i = threadIdx.x; i < N * cols; i += NT) {
             const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
@@ -550,8 +559,13 @@ template <typename R> struct Tiles;
 // 128-byte row segments; two workgroups, 32 waves, per CU); final pass:
 // 512 threads of EF elements, half the tile, so that three workgroups
 // share a CU while each walks its species one after another
-template <> struct Tiles<float> { static constexpr int EY = 8, EF = 8; };
-template <> struct Tiles<double> { static constexpr int EY = 4, EF = 4; };
+// RY: arithmetic of the y pass.  FP32 data keeps FP32 arithmetic: an FP64
+// y pass (8 columns of 512 rows of double2, two workgroups per CU) measured
+// 0.68 against 0.37 ms per 16-frame C2 launch, for errors of the FP32 form
+// that stay of the same order (eps32 x the Bragg-to-diffuse amplitude ratio
+// of near-lattice inputs, whichever pass order)
+template <> struct Tiles<float> { static constexpr int EY = 8, EF = 8; using RY = float; };
+template <> struct Tiles<double> { static constexpr int EY = 4, EF = 4; using RY = double; };
 constexpr int colfft_nt(bool final) { return final ? 512 : 1024; }
 
 
@@ -652,7 +666,8 @@ template <typename R> int colfft_tile_elems(bool final)
 // LDS of one column pass of a job: tile + twiddles (+ species sum for the final pass)
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
 {
-    return ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) * sizeof(typename cplx<R>::T);
+    const size_t es = final ? sizeof(typename cplx<R>::T) : sizeof(typename cplx<typename Tiles<R>::RY>::T);
+    return ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) * es;
 }
 
 template <typename R>
@@ -661,20 +676,21 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
 {
     if (nblocks <= 0) return;
     constexpr int EF = Tiles<R>::EF, EY = Tiles<R>::EY, NTF = colfft_nt(true), NTY = colfft_nt(false);
+    using RY = typename Tiles<R>::RY;
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, EF, NTF, true>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, R, EF, NTF, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, EY, NTY, false>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, RY, EY, NTY, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         attr = true;
     }
     if (final)
-        hipLaunchKernelGGL((k_colfft<R, EF, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
+        hipLaunchKernelGGL((k_colfft<R, R, EF, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
                            njobs);
     else
-        hipLaunchKernelGGL((k_colfft<R, EY, NTY, false>), dim3((unsigned)nblocks), dim3(NTY), lds, st, jobs, blk0s,
-                           njobs);
+        hipLaunchKernelGGL((k_colfft<R, RY, EY, NTY, false>), dim3((unsigned)nblocks), dim3(NTY), lds, st, jobs,
+                           blk0s, njobs);
 }
 
 #define MPSQ_INST(R)                                                                                                  \

@@ -74,18 +74,25 @@ constexpr int NARROW_MAX = 256;  // largest covering bound of a 32-bit (narrow) 
 // counters after the 8 per-XCD tile counters: fallback items queued, taken
 constexpr int CTR_FB_COUNT = 8 * XCD_CTR_STRIDE, CTR_FB_TAKE = 9 * XCD_CTR_STRIDE;
 
-// phi(t) = exp(beta (sqrt(1 - ic t^2) - 1)), |t| <= w/2.  FP32: hardware
-// exp2 / sqrt (relative error ~1e-6 only where phi < 1e-6, i.e. absolute
-// error < 1e-12) with bl = beta * log2(e), so that the exponent is one fma;
-// FP64: the accurate library functions with bl = beta.
-__device__ __forceinline__ float es_kernel(float t, float bl, float ic)
+// phi(t) = exp(beta (sqrt(1 - u^2) - 1)) with u = 2t/w, |u| <= 1.  FP32:
+// the exponent is written without the cancellation of sqrt(1 - u^2) - 1,
+//   beta (sqrt(1 - u^2) - 1) = -beta u^2 / (1 + sqrt(1 - u^2)),
+// so its relative error is a few ulp everywhere (the direct form loses
+// ~1e-6 of phi near the kernel's centre, which shows as ~1e-5 errors on the
+// weak modes of near-lattice inputs); then the hardware exp2 with
+// nbl = -beta log2(e).  A stencil's cells lie on the support (|u| <= 1), so
+// a rounding of 1 - u^2 below zero is clamped to the support's edge.
+__device__ __forceinline__ float es_kernel_uu(float uu, float nbl)
 {
-    const float a = 1.0f - t * t * ic;
-    return (a > 0.0f) ? __builtin_amdgcn_exp2f(fmaf(__builtin_amdgcn_sqrtf(a), bl, -bl)) : 0.0f;
+    const float s = __builtin_amdgcn_sqrtf(fmaxf(1.0f - uu, 0.0f));
+    return __builtin_amdgcn_exp2f((nbl * uu) * __builtin_amdgcn_rcpf(1.0f + s));
 }
-// the kernel's shape argument: beta * log2(e) for FP32, beta for FP64
+__device__ __forceinline__ float es_kernel(float t, float nbl, float ic) { return es_kernel_uu(t * t * ic, nbl); }
+// phi at u = t * 2 / w
+__device__ __forceinline__ float es_kernel_u(float u, float nbl) { return es_kernel_uu(u * u, nbl); }
+// the kernel's shape argument: -beta * log2(e) for FP32, beta for FP64
 template <typename R> __host__ __device__ inline R es_shape(double beta);
-template <> __host__ __device__ inline float es_shape<float>(double beta) { return (float)(beta * 1.4426950408889634); }
+template <> __host__ __device__ inline float es_shape<float>(double beta) { return (float)(-beta * 1.4426950408889634); }
 template <> __host__ __device__ inline double es_shape<double>(double beta) { return beta; }
 __device__ __forceinline__ double es_kernel(double t, double beta, double ic)
 {
@@ -148,16 +155,43 @@ template <typename R> int spread_group_rows(int rows, int n1)
     while (2 * g <= rows && 2 * g * n1 <= te) g *= 2;
     return g;
 }
-template <typename R> size_t spread_lds_bytes(int rows, int n1, int w)
+// bytes of the tile region (accumulators / FFT rows); the twiddles follow it
+template <typename R> size_t spread_tile_bytes(int rows, int n1, int w)
 {
     const size_t G = (size_t)spread_group_rows<R>(rows, n1);
     const size_t RSX = (size_t)spread_row_stride_x(n1, w);
     const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * 8 : 0;
     const size_t wideb = std::max(2 * G * spread_row_stride(n1, w) * 8, G * RSX * sizeof(typename cplx<R>::T));
-    return (std::max(narrowb, wideb) + 15) / 16 * 16 + (size_t)n1 * sizeof(typename cplx<R>::T);
+    return (std::max(narrowb, wideb) + 15) / 16 * 16;
 }
+template <typename R> size_t spread_lds_bytes(int rows, int n1, int w)
+{
+    // + n1 twiddles; FP32: + the tile's scaled deconvolution factors (N1 <= n1 / 2
+    // floats) and its rows' output offsets (rows int64)
+    const size_t extra = sizeof(R) == 4 ? (size_t)(n1 / 2) * 4 + (size_t)rows * 8 : 0;
+    return spread_tile_bytes<R>(rows, n1, w) + (size_t)n1 * sizeof(typename cplx<R>::T) + extra;
+}
+// byte offset of the narrow kernel's per-tile tables after the twiddles
+__host__ __device__ inline int spread_tab_offset(int lds_tw, int n1) { return lds_tw + n1 * 8; }
 
 constexpr double FIX_MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+// Narrow cells from two FP64 fmas.  fma(v, k, FIX_MAGIC) is FIX_MAGIC +
+// round(v k) exactly for |v k| < 2^51, and as a 64-bit pattern that is
+// 0x4338000000000000 + round(v k): its low word is re, its high word
+// 0x43380000 plus re's sign extension.  With FIX_MAGIC_IM = FIX_MAGIC +
+// 0xBCC80000 the low word of the imaginary fma is im + 0xBCC80000 (mod 2^32),
+// and 0x43380000 + 0xBCC80000 = 2^32, so
+//   packed = {lo: lo(dr), hi: hi(dr) + lo(di)} = im * 2^32 + re (re sign-extended)
+// -- two fmas and one 32-bit add per cell.
+constexpr double FIX_MAGIC_IM = 6755402608279552.0;  // 1.5 * 2^52 + 0xBCC80000
+// {lo: lo(br), hi: hi(br) + lo(bi)}: the 32-bit add is written as one
+// instruction (left to itself the compiler forms a 64-bit add plus a move)
+__device__ __forceinline__ unsigned long long pack_cell(unsigned long long br, unsigned long long bi)
+{
+    unsigned hi;
+    asm("v_add_u32 %0, %1, %2" : "=v"(hi) : "v"((unsigned)(br >> 32)), "v"((unsigned)bi));
+    return ((unsigned long long)hi << 32) | (unsigned)br;
+}
 
 // One Stockham stage of `cnt` row transforms of length n (row-major) whose
 // input element i of row t comes from load(t, i) and whose output element f
@@ -299,7 +333,8 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
     int64_t b0 = 0;
     int cnt = 0;
     if (lane < nruns)
-        tile_run(g, J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1), ylo, nyb, zlo, lane, b0, cnt);
+        tile_run(g, J.binstart + (int64_t)t * J.bin_tstride * (g.nbins + 1), ylo, nyb, zlo, lane, b0, cnt,
+                 [](const int64_t *q) { return gload(q); });
     int cb = 0x7fffffff;
     if (bound && J.binmax) {
         const int *bm = J.binmax + (int64_t)t * J.bin_tstride * (g.nbins + 1);
@@ -307,7 +342,7 @@ __device__ __forceinline__ void runs_publish(RunTable &S, int pb, const SpreadJo
         int sum = 0;
         for (int i = lane; i < nv; i += 64) {
             const int k = i / nyb, jy = i - k * nyb;
-            sum += bm[posmod(zlo + k, g.nbZ) * g.nbY + posmod(ylo + jy, g.nbY)];
+            sum += gload(bm + posmod(zlo + k, g.nbZ) * g.nbY + posmod(ylo + jy, g.nbY));
         }
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
@@ -354,8 +389,8 @@ template <typename R> struct Cand {
                 atomicOr(J.err, 2);
                 return;
             }
-            pr = J.srec[p];
-            cv = J.cs[(int64_t)tt * J.c_tstride + p];
+            pr = gload(J.srec + p);
+            cv = gload(J.cs + (int64_t)tt * J.c_tstride + p);
         }
     }
 };
@@ -382,7 +417,7 @@ template <typename R, int NT> __device__ __forceinline__ void load_twiddles(cons
 {
     using C = typename cplx<R>::T;
     C *tw = reinterpret_cast<C *>(smem + J.lds_tw);
-    for (int i = threadIdx.x; i < J.g.n1; i += NT) tw[i] = J.tw1[i];
+    for (int i = threadIdx.x; i < J.g.n1; i += NT) tw[i] = gload(J.tw1 + i);
 }
 
 // ---------------------------------------------------------------------------
@@ -445,7 +480,10 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         const int by = (g.dim >= 2) ? g.by : 1;
         const int bz = (g.dim >= 3) ? g.bz : 1;
         const int rows = by * bz;
-        const R beta = es_shape<R>(g.beta), ic = (R)g.ic;
+        const float bl = es_shape<R>(g.beta);      // -beta * log2(e)
+        const float sw = (float)(2.0 / (double)w);  // u = t * 2/w
+        const int nyb = min(W, by);                 // kernel values along y a tile can use
+        const int RSX8 = RSX * 8;                   // bytes per accumulator row
         const C *tw = reinterpret_cast<const C *>(smem_raw + J.lds_tw);
         const float am = S.amax[pb];
         const int cb = S.cb[pb];
@@ -489,36 +527,42 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 if (g.dim >= 2) overlap(p.ly, y0, by, w, g.n2, fy, ny, my);
                 if (g.dim >= 3) overlap(p.lz, z0, bz, w, g.n3, fz, nz, mz);
                 if (ny <= 0 || nz <= 0) continue;
-                R kx[W];
+                // kernel values at u = (d + m) * 2/w: x for all W cells (FP64
+                // for the cell fmas), y only for the tile's rows (nyb <= by,
+                // uniform), z one per z row inside the row loop
+                double kxd[W];
+                const float ux = p.dx * sw;
 #pragma unroll
-                for (int m = 0; m < W; ++m) kx[m] = es_kernel(p.dx + (R)m, beta, ic);
-                const R dy0 = p.dy + (R)my, dz0 = p.dz + (R)mz;  // offsets of the first covered rows
-                R ky[W];
+                for (int m = 0; m < W; ++m) kxd[m] = (double)es_kernel_u(fmaf((float)m, sw, ux), bl);
+                float ky[W];
+                const float uy = (p.dy + (float)my) * sw;
 #pragma unroll
-                for (int iy = 0; iy < W; ++iy)
-                    ky[iy] = (g.dim >= 2) ? ((iy < ny) ? es_kernel(dy0 + (R)iy, beta, ic) : (R)0) : (R)1;
-                R kz[W];
-#pragma unroll
-                for (int iz = 0; iz < W; ++iz)
-                    kz[iz] = (g.dim >= 3) ? ((iz < nz) ? es_kernel(dz0 + (R)iz, beta, ic) : (R)0) : (R)1;
+                for (int iy = 0; iy < W; ++iy) {
+                    ky[iy] = 0.0f;
+                    if (iy < nyb) ky[iy] = (g.dim < 2) ? 1.0f : (iy < ny) ? es_kernel_u(fmaf((float)iy, sw, uy), bl) : 0.0f;
+                }
+                const float uz = (p.dz + (float)mz) * sw;
                 const float csx = ldexpf(cc.x, q), csy = ldexpf(cc.y, q);  // strength * 2^q
+                const int abase = (int)(p.lx * 8u);                        // byte offset of the first cell
                 for (int iz = 0; iz < nz; ++iz) {
-                    const R wz = kz[iz];  // iz is uniform over the lanes still in the loop
-                    const int rr0 = (fz + iz) * by + fy;  // tile row of iy = 0
+                    const float wz = (g.dim >= 3) ? es_kernel_u(fmaf((float)iz, sw, uz), bl) : 1.0f;
+                    const int rr0 = __mul24(fz + iz, by) + fy;  // tile row of iy = 0
 #pragma unroll
                     for (int iy = 0; iy < W; ++iy) {
-                        if (iy < ny) {
-                            const int r = rr0 + iy;
-                            const R wyz = ky[iy] * wz;
-                            const float sr = csx * wyz, si = csy * wyz;
-                            unsigned long long *ac = acc + r * RSX + p.lx;
+                        if (iy < nyb && iy < ny) {  // iy < nyb is uniform
+                            const float wyz = ky[iy] * wz;
+                            const double vr = (double)(csx * wyz), vi = (double)(csy * wyz);
+                            unsigned char *ab = reinterpret_cast<unsigned char *>(acc) +
+                                                (__mul24(rr0 + iy, RSX8) + abase);
+                            unsigned long long *ac = reinterpret_cast<unsigned long long *>(ab);
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
-                                const int re = rint_up(sr * kx[m]), im = rint_up(si * kx[m]);
-                                // im * 2^32 + re with re sign-extended: one 64-bit add per cell
-                                const unsigned long long pk =
-                                    ((unsigned long long)(unsigned)(im + (re >> 31)) << 32) | (unsigned)re;
-                                __hip_atomic_fetch_add(ac + m, pk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                const unsigned long long br =
+                                    (unsigned long long)__double_as_longlong(__builtin_fma(vr, kxd[m], FIX_MAGIC));
+                                const unsigned long long bi =
+                                    (unsigned long long)__double_as_longlong(__builtin_fma(vi, kxd[m], FIX_MAGIC_IM));
+                                __hip_atomic_fetch_add(ac + m, pack_cell(br, bi), __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
                     }
@@ -542,19 +586,40 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         // ---- x FFT of all rows at once (2E elements per thread): the first
         // stage reads every group's accumulators before any row is written
         if (ok) {
+            // per tile: the kept frequencies' factors 2^-q / phi_hat and the
+            // G1 offset of every tile row (read by the last stage, after at
+            // least one barrier of the FFT)
+            const int N1 = g.N1, h1 = N1 / 2;
+            float *phs = reinterpret_cast<float *>(smem_raw + spread_tab_offset(J.lds_tw, n1));
+            int64_t *rowoff = reinterpret_cast<int64_t *>(phs + n1 / 2);
+            for (int i = threadIdx.x; i < N1; i += NT) phs[i] = ldexpf(gload(J.phinv1 + i), -q);
+            for (int r = threadIdx.x; r < rows; r += NT) {
+                const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
+                rowoff[r] = (((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1;
+            }
+            if (J.rp1.nst == 1) __syncthreads();
             // the ghost cells x = n1 + i fold onto i < w here (32-bit adds
             // wrap; the final sums are exact); 2^-q goes into the output factor
             auto load_acc = [&](int row, int i) -> C {
-                const int a = row * RSX + i;
+                const int a = __mul24(row, RSX) + i;
                 unsigned long long T = acc[a];
                 if (i < W) T += acc[a + n1];
                 const int re = (int)(unsigned)T;                // |sum| < 2^31: exact
                 const int im = (int)(unsigned)(T >> 32) - (re >> 31);  // undo re's sign extension
                 return cmk<C>((float)re, (float)im);
             };
-            auto load_tile = [&](int row, int i) -> C { return tile[row * RSX + padx<true>(i)]; };
-            auto store_tile = [&](int row, int i, C v) { tile[row * RSX + padx<true>(i)] = v; };
-            auto store_out = [&](int row, int f, C v) { store_kept(J, t, y0, z0, by, row, f, v, -q); };
+            auto load_tile = [&](int row, int i) -> C { return tile[__mul24(row, RSX) + padx<true>(i)]; };
+            auto store_tile = [&](int row, int i, C v) { tile[__mul24(row, RSX) + padx<true>(i)] = v; };
+            const bool cj = J.final_conj != 0;
+            C *G1 = J.G1;
+            auto store_out = [&](int row, int f, C v) {
+                int kk;
+                if (f < N1 - h1) kk = f + h1;
+                else if (f >= n1 - h1) kk = f - (n1 - h1);
+                else return;
+                const float sc = phs[kk];
+                gstore(G1 + rowoff[row] + kk, cmk<C>(v.x * sc, cj ? -(v.y * sc) : v.y * sc));
+            };
             x_fft<C, 2 * E, NT>(J.rp1, n1, rows, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
@@ -786,6 +851,7 @@ int spread_max_runs() { return MAX_RUNS; }
 int spread_max_tile_b() { return MAX_TILE_B; }
 int spread_max_wide_passes() { return MAX_WIDE_PASSES; }
 template <typename R> size_t spread_lds(int rows, int n1, int w) { return spread_lds_bytes<R>(rows, n1, w); }
+template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w) { return spread_tile_bytes<R>(rows, n1, w); }
 
 __global__ void k_zero_i32(int *__restrict__ p, int n)
 {
@@ -873,6 +939,7 @@ void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, 
     template int spread_max_rows<R>(int);                                                                             \
     template int spread_group_rows<R>(int, int);                                                                      \
     template size_t spread_lds<R>(int, int, int);                                                                     \
+    template size_t spread_lds_twiddles<R>(int, int, int);                                                            \
     template void launch_spread_jobs<R>(const SpreadJob<R> *, const int *, int, int, const int *, int *, int *,      \
                                         size_t, hipStream_t);
 

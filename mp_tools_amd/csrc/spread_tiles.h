@@ -64,8 +64,13 @@ __host__ __device__ inline int nruns_of(const Geo &g, int y0, int z0)
 
 // Run `r` (< 2 * nzb) of a tile: the sorted positions [b0, b0 + cnt) of its
 // bins, from the transform's bin starts bs (nbins + 1 entries).
+struct PlainLoad {
+    __host__ __device__ int64_t operator()(const int64_t *q) const { return *q; }
+};
+// ld(q) reads bs entries (the kernels pass a global-address-space load)
+template <class Ld = PlainLoad>
 __host__ __device__ inline void tile_run(const Geo &g, const int64_t *bs, int ylo, int nyb, int zlo, int r,
-                                         int64_t &b0, int &cnt)
+                                         int64_t &b0, int &cnt, Ld ld = Ld())
 {
     b0 = 0;
     cnt = 0;
@@ -94,8 +99,8 @@ __host__ __device__ inline void tile_run(const Geo &g, const int64_t *bs, int yl
         yb = b;
     }
     if (yb > ya) {
-        b0 = bs[zb * g.nbY + ya];
-        cnt = (int)(bs[zb * g.nbY + yb] - b0);
+        b0 = ld(bs + zb * g.nbY + ya);
+        cnt = (int)(ld(bs + zb * g.nbY + yb) - b0);
     }
 }
 

@@ -239,6 +239,68 @@ void launch_xray_table(const int64_t *N, const double *qs, int nsp, const double
                        qs[1], qs[2], nsp, coef, w);
 }
 
+// mp_sqom's per-Q tables of a Q plane, in the reference's REAL(4)
+// arithmetic without contraction (mp_sqom56.f90:909-936): for pixel (i, j)
+// (i fastest, i = k1 + n_qx/2 of the amplitudes' CMCL order)
+//   q1 = (qx_min + i * dqx) e1,  q2 = (qy_min + j * dqy) e2   (r.l.u.)
+//   q2_norm = 1 / sqrt(|q1 + q2|^2 + |q_v|^2)   (1 where that is below 1e-8)
+//   x_ffq[s] = c_s + sum_k a_sk exp(-0.25 b_sk q_sq'),
+//   q_sq' = 0.25 (|q1' + q2'|^2 + |q_vp|^2) with the 1/A vectors e1p, e2p, q_vp
+__device__ __forceinline__ float dot3_nc(const float *a, const float *b)
+{
+#pragma clang fp contract(off)
+    const float x = a[0] * b[0];
+    const float y = a[1] * b[1];
+    const float z = a[2] * b[2];
+    return (x + y) + z;
+}
+
+__global__ void k_qtables(QTableParams p, float *__restrict__ q2_norm, float *__restrict__ x_ffq)
+{
+#pragma clang fp contract(off)
+    const int64_t np = (int64_t)p.n_qx * p.n_qy;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int i = (int)(q % p.n_qx), j = (int)(q / p.n_qx);
+    const float ax = p.qx_min + (float)i * p.dqx;
+    const float ay = p.qy_min + (float)j * p.dqy;
+    if (q2_norm) {
+        float v[3];
+        for (int d = 0; d < 3; ++d) {
+            const float q1 = ax * p.e1[d];
+            const float q2 = ay * p.e2[d];
+            v[d] = q1 + q2;
+        }
+        const float q_sq = dot3_nc(v, v) + dot3_nc(p.q_v, p.q_v);
+        q2_norm[q] = (q_sq >= 1.0e-8f) ? 1.0f / sqrtf(q_sq) : 1.0f;
+    }
+    if (x_ffq) {
+        float v[3];
+        for (int d = 0; d < 3; ++d) {
+            const float q1 = ax * p.e1p[d];
+            const float q2 = ay * p.e2p[d];
+            v[d] = q1 + q2;
+        }
+        const float q_sq = 0.25f * (dot3_nc(v, v) + dot3_nc(p.q_vp, p.q_vp));
+        for (int s = 0; s < p.nsp; ++s) {
+            float f = p.ff[s][8];
+            for (int k = 0; k < 4; ++k) {
+                const float e = expf(-0.25f * p.ff[s][2 * k + 1] * q_sq);
+                const float t = p.ff[s][2 * k] * e;
+                f = f + t;
+            }
+            x_ffq[(int64_t)s * np + q] = f;
+        }
+    }
+}
+
+void launch_qtables(const QTableParams &p, float *q2_norm, float *x_ffq, hipStream_t st)
+{
+    const int64_t np = (int64_t)p.n_qx * p.n_qy;
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_qtables, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, p, q2_norm, x_ffq);
+}
+
 template <typename R> struct SqwTiles;
 template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
 template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };

@@ -92,7 +92,8 @@ def make_spec(dim, n_modes, species_counts, eps=1e-6, precision="fp32", iflag=-1
     s.opts.tile_by, s.opts.tile_bz = int(tile_by), int(tile_bz)
     info = dict(dim=dim, n_modes=n_modes, NQ=NQ, precision=precision, fp64=fp64,
                 cdtype=torch.complex128 if fp64 else torch.complex64, nsp=nsp, n_tot=n_tot,
-                combine=bool(combine))
+                combine=bool(combine),
+                Mact=int(counts.sum() if mask is None else counts[np.asarray(mask) != 0].sum()))
     return s, keep, info
 
 
@@ -160,6 +161,30 @@ class SqwEngine:
                                      out_tot.data_ptr())
         _lib.check("sqw_reduce", rc)
         return out, out_tot
+
+    def debug_sorted(self, pos, frame=0):
+        """Staged, bin-sorted points of one frame as the spread reads them:
+        (atom index in the frame, first kernel cell [n, 3], selection flag),
+        int32 numpy arrays in sorted order."""
+        import numpy as np
+
+        if not pos.is_cuda:
+            raise ValueError("pos must be a CUDA float32 tensor [T, n_tot, 4]")
+        pos = pos.contiguous()
+        n = int(self.Mact)
+        atom = np.zeros(n, np.int32)
+        cell = np.zeros((n, 3), np.int32)
+        keep = np.zeros(n, np.int32)
+        rc = self._L.mpsq_sqw_debug_sorted(self._h, pos.data_ptr(), int(frame), atom.ctypes.data,
+                                           cell.ctypes.data, keep.ctypes.data, n)
+        _lib.check("sqw_debug_sorted", rc)
+        return atom, cell, keep
+
+    def info(self):
+        """Fine grid, kernel width, spread tile and sort-bin shape."""
+        out = (C.c_int64 * 8)()
+        _lib.check("sqw_info", self._L.mpsq_sqw_info(self._h, out))
+        return dict(zip(("n1", "n2", "n3", "w", "by", "bz", "BY", "BZ"), [int(v) for v in out]))
 
     def set_timing(self, on=True):
         """Record HIP events around every launch group (per-stage times)."""
@@ -322,9 +347,12 @@ def qplane(n_row, bz_n: float, e1=(1, 0, 0), e2=(0, 1, 0), q_center=(0.0, 0.0, 0
         tpe1, tpe2, tpev = (f(np.float64(twopi) / np.float64(f(np.dot(v, span)))) for v in (e1, e2, ev))
     ainv = np.eye(3, dtype=f) if a_cell_inv is None else np.asarray(a_cell_inv, f)
     bz_n = f(bz_n)
+    # plane vectors in 1/A (mp_sqom56.f90:812-820): 2 pi a_cell_inv e for lattices
     if nsuper > 1:
-        e1p = (twopi * (ainv @ e1).astype(np.float64)).astype(f)
-        e2p = (twopi * (ainv @ e2).astype(np.float64)).astype(f)
+        e1p, e2p, evp = ((twopi * (ainv @ v).astype(np.float64)).astype(f) for v in (e1, e2, ev))
+    else:
+        e1p, e2p, evp = ((ainv @ v).astype(f) for v in (e1, e2, ev))
+    if nsuper > 1:
         e1pn, e2pn = f(np.sqrt(np.dot(e1p, e1p))), f(np.sqrt(np.dot(e2p, e2p)))
         cosa = f(np.dot(e1p, e2p) / (e1pn * e1pn))
         bz_ny = max(bz_n, f(bz_n * (e1pn / (e2pn * f(np.sqrt(f(1) - cosa * cosa))))))
@@ -345,7 +373,45 @@ def qplane(n_row, bz_n: float, e1=(1, 0, 0), e2=(0, 1, 0), q_center=(0.0, 0.0, 0
         qx_min, qx_max = f(qx0 - f(0.5) * bz_nx), f(qx0 + f(0.5) * bz_nx)
     if n_qy > 1:
         qy_min, qy_max = f(qy0 - f(0.5) * bz_ny), f(qy0 + f(0.5) * bz_ny)
+    # vertical components of q_center (:911, :920), in r.l.u. and in 1/A
+    q_v = (f(np.dot(ev, qc) / nv) * ev).astype(f)
+    evpn = f(np.sqrt(np.dot(evp, evp)))
+    q_vp = (f(np.dot(evp, qc) / evpn) * evp).astype(f) if evpn > 0 else np.zeros(3, f)
     return dict(e1=e1, e2=e2, ev=ev, tpe=(tpe1, tpe2, tpev), bz_nx=bz_nx, bz_ny=bz_ny, n_qx=n_qx, n_qy=n_qy,
                 qx_range=(qx_min, qx_max), qy_range=(qy_min, qy_max), n_modes=(n_qx, n_qy),
+                e1p=e1p, e2p=e2p, evp=evp, q_v=q_v, q_vp=q_vp,
                 engine_kw=dict(axes=np.stack([e1, e2, ev]).astype(np.float64),
                                tpe=tuple(float(t) for t in (tpe1, tpe2, tpev)), q0=tuple(float(v) for v in qc)))
+
+
+def qtables(plane, x_ffpar=None, stream=None, device=0):
+    """The reference's per-Q tables of a Q plane from qplane(), computed on
+    the device (mpsq_qtables; mp_sqom56.f90:909-936): q2_norm [n_qx * n_qy]
+    (1/|Q| in r.l.u., the factor on C' of the S(Q)/Q^2 scaling) and, when
+    x_ffpar ([nsp, 9]: a1, b1, a2, b2, a3, b3, a4, b4, c per species, as the
+    reference reads them) is given, the X-ray weights x_ffq [nsp, n_qx *
+    n_qy]; CUDA float32 tensors in the CMCL order of the plane's amplitudes."""
+    import torch
+
+    L = _lib.load()
+    q = _lib.QPlane()
+    q.n_qx, q.n_qy = int(plane["n_qx"]), int(plane["n_qy"])
+    q.qx_min, q.qy_min = float(plane["qx_range"][0]), float(plane["qy_range"][0])
+    q.bz_nx, q.bz_ny = float(plane["bz_nx"]), float(plane["bz_ny"])
+    for d in range(3):
+        q.e1[d], q.e2[d], q.q_v[d] = float(plane["e1"][d]), float(plane["e2"][d]), float(plane["q_v"][d])
+        q.e1p[d], q.e2p[d], q.q_vp[d] = float(plane["e1p"][d]), float(plane["e2p"][d]), float(plane["q_vp"][d])
+    npix = q.n_qx * q.n_qy
+    dev = torch.device("cuda", int(device))
+    q2 = torch.empty(npix, dtype=torch.float32, device=dev)
+    ff, xq, nsp = None, None, 0
+    if x_ffpar is not None:
+        ff = np.ascontiguousarray(x_ffpar, np.float32).reshape(-1, 9)
+        nsp = ff.shape[0]
+        xq = torch.empty((nsp, npix), dtype=torch.float32, device=dev)
+    if stream is None:
+        stream = torch.cuda.current_stream(dev).cuda_stream
+    rc = L.mpsq_qtables(C.byref(q), nsp, None if ff is None else ff.ctypes.data, q2.data_ptr(),
+                        None if xq is None else xq.data_ptr(), stream)
+    _lib.check("qtables", rc)
+    return q2, xq

@@ -20,6 +20,13 @@
  *                           linearity; 3D == stack of 2D slices with phase.
  *   orc_dirft_type1_sampled equals orc_dirft_type1 at the sampled modes and
  *                           a numpy brute force on tiny inputs.
+ *   orc_dirft_type1_prod    equals orc_dirft_type1_sampled on every mode of
+ *                           the product set, and orc_dirft_type1 where the
+ *                           set is the whole mode box.
+ *   orc_qtables             closed forms on an axis-aligned plane (1/|Q|,
+ *                           the Q = 0 pixel, the vertical component), the
+ *                           form factors at Q = 0 against the atomic numbers
+ *                           of the reference's table, single-Gaussian decay.
  *   orc_fold / orc_counting_sort  closed-form fold checked in numpy (floor /
  *                           ceil / modulo) and numpy's stable argsort.
  *   orc_stage               numpy restatement of the dot products + the
@@ -210,6 +217,137 @@ int orc_dirft_type1_sampled(int dim, int64_t M, const double *x,
         free(acc);
         for (int d = 0; d < 3; ++d) free(tab[d]);
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a8: mp_sqom's per-Q tables of a Q plane (mp_sqom56.f90:909-936), REAL(4)  */
+/* arithmetic without contraction, pixel (i, j) with i fastest:              */
+/*   q1 = (qx_min + i*(bz_nx/n_qx))*e1, q2 = (qy_min + j*(bz_ny/n_qy))*e2    */
+/*   q_sq = dot(q1+q2, q1+q2) + dot(q_v, q_v)                                */
+/*   q2_norm = 1/sqrt(q_sq) if q_sq >= 1e-8 else 1                    (:914) */
+/*   q_sq' = .25*(dot(q1'+q2', q1'+q2') + dot(q_vp, q_vp)) on e1p, e2p (:925)*/
+/*   x_ffq(s) = ff(9) + sum_k ff(2k-1)*exp(-.25*ff(2k)*q_sq')     (:929-931) */
+/* ff: [nsp][9] as read from the .par file (a1,b1,a2,b2,a3,b3,a4,b4,c).     */
+/* ------------------------------------------------------------------------ */
+static float orc_dot3f(const float *a, const float *b)
+{
+    const float x = a[0] * b[0];
+    const float y = a[1] * b[1];
+    const float z = a[2] * b[2];
+    return (x + y) + z;
+}
+
+void orc_qtables(int n_qx, int n_qy, float qx_min, float qy_min, float bz_nx, float bz_ny,
+                 const float *e1, const float *e2, const float *q_v, const float *e1p,
+                 const float *e2p, const float *q_vp, int nsp, const float *ff,
+                 float *q2_norm, float *x_ffq)
+{
+    const float dqx = bz_nx / (float)n_qx, dqy = bz_ny / (float)n_qy;
+    const int64_t np = (int64_t)n_qx * n_qy;
+    for (int j = 0; j < n_qy; ++j)
+        for (int i = 0; i < n_qx; ++i) {
+            const int64_t q = (int64_t)j * n_qx + i;
+            const float ax = qx_min + (float)i * dqx;
+            const float ay = qy_min + (float)j * dqy;
+            float v[3];
+            for (int d = 0; d < 3; ++d) {
+                const float a = ax * e1[d];
+                const float b = ay * e2[d];
+                v[d] = a + b;
+            }
+            const float q_sq = orc_dot3f(v, v) + orc_dot3f(q_v, q_v);
+            if (q2_norm) q2_norm[q] = (q_sq >= 1.0e-8f) ? 1.0f / sqrtf(q_sq) : 1.0f;
+            if (!x_ffq) continue;
+            for (int d = 0; d < 3; ++d) {
+                const float a = ax * e1p[d];
+                const float b = ay * e2p[d];
+                v[d] = a + b;
+            }
+            const float qs = 0.25f * (orc_dot3f(v, v) + orc_dot3f(q_vp, q_vp));
+            for (int s = 0; s < nsp; ++s) {
+                const float *c = ff + 9 * s;
+                float f = c[8];
+                for (int k = 0; k < 4; ++k) {
+                    const float e = expf(-0.25f * c[2 * k + 1] * qs);
+                    const float t = c[2 * k] * e;
+                    f = f + t;
+                }
+                x_ffq[(int64_t)s * np + q] = f;
+            }
+        }
+}
+
+/* Direct sum on a product set of modes: F[a3][a2][a1] at (k1s[a1], k2s[a2], */
+/* k3s[a3]) (a1 fastest), the same definition as orc_dirft_type1.  Per point  */
+/* the n1 + n2 + n3 one-dimensional exponentials are exact sincos values and  */
+/* each mode costs one complex multiply-add, so a few hundred to ten thousand */
+/* seeded random modes over millions of points stay cheap.  Threads sum       */
+/* private accumulators over static point ranges, added in thread order.     */
+int orc_dirft_type1_prod(int dim, int64_t M, const double *x, const double *y,
+                         const double *z, const double *c, int iflag,
+                         int64_t n1, const int64_t *k1s, int64_t n2,
+                         const int64_t *k2s, int64_t n3, const int64_t *k3s,
+                         double *f)
+{
+    if (dim < 1 || dim > 3 || M < 0 || n1 < 1 || n2 < 1 || n3 < 1) return 2;
+    if ((dim < 2 && n2 != 1) || (dim < 3 && n3 != 1)) return 2;
+    const double s = (iflag >= 0) ? 1.0 : -1.0;
+    const int64_t K = n1 * n2 * n3;
+    for (int64_t i = 0; i < 2 * K; ++i) f[i] = 0.0;
+    const int nth = orc_num_threads();
+    double *part = (double *)calloc((size_t)(2 * K) * (size_t)nth, sizeof(double));
+    if (!part) return 2;
+#pragma omp parallel num_threads(nth)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *ar = part + (size_t)(2 * K) * (size_t)tid, *ai = ar + K;
+        double *e1r = (double *)malloc(sizeof(double) * (size_t)(2 * n1));
+        double *e1i = e1r + n1;
+        double e2r[4096], e2i[4096], e3r[4096], e3i[4096];
+        const int64_t lo = M * tid / nth, hi = M * (tid + 1) / nth;
+        for (int64_t j = lo; j < hi; ++j) {
+            for (int64_t a = 0; a < n1; ++a) {
+                const double ph = s * (double)k1s[a] * x[j];
+                e1r[a] = cos(ph);
+                e1i[a] = sin(ph);
+            }
+            for (int64_t a = 0; a < n2 && a < 4096; ++a) {
+                const double ph = dim >= 2 ? s * (double)k2s[a] * y[j] : 0.0;
+                e2r[a] = cos(ph);
+                e2i[a] = sin(ph);
+            }
+            for (int64_t a = 0; a < n3 && a < 4096; ++a) {
+                const double ph = dim >= 3 ? s * (double)k3s[a] * z[j] : 0.0;
+                e3r[a] = cos(ph);
+                e3i[a] = sin(ph);
+            }
+            const double cr = c[2 * j], ci = c[2 * j + 1];
+            for (int64_t a3 = 0; a3 < n3; ++a3) {
+                const double ur = cr * e3r[a3] - ci * e3i[a3], ui = cr * e3i[a3] + ci * e3r[a3];
+                for (int64_t a2 = 0; a2 < n2; ++a2) {
+                    const double tr = ur * e2r[a2] - ui * e2i[a2], ti = ur * e2i[a2] + ui * e2r[a2];
+                    double *rr = ar + (a3 * n2 + a2) * n1, *ri = ai + (a3 * n2 + a2) * n1;
+                    for (int64_t a = 0; a < n1; ++a) {
+                        rr[a] += tr * e1r[a] - ti * e1i[a];
+                        ri[a] += tr * e1i[a] + ti * e1r[a];
+                    }
+                }
+            }
+        }
+        free(e1r);
+    }
+    for (int t = 0; t < nth; ++t) {
+        const double *ar = part + (size_t)(2 * K) * (size_t)t, *ai = ar + K;
+        for (int64_t m = 0; m < K; ++m) {
+            f[2 * m] += ar[m];
+            f[2 * m + 1] += ai[m];
+        }
+    }
+    free(part);
     return 0;
 }
 
@@ -512,8 +650,28 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
     int32_t *l = (int32_t *)malloc((size_t)(3 * M) * sizeof(int32_t));
     double *d = (double *)malloc((size_t)(3 * M) * sizeof(double));
     int32_t *key = (int32_t *)malloc((size_t)M * sizeof(int32_t));
+    /* kernel values of every point, computed once: kv[(3 j + dd) w + m] */
+    double *kv = (double *)malloc((size_t)M * 3 * (size_t)w * sizeof(double));
     const int64_t nn[3] = {n1, n2, n3};
     orc_fold(dim, M, x, y, z, nn, w, 1, 1, l, d, key);
+    const double hw = 0.5 * w, ic = 1.0 / (hw * hw);
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < M; ++j) {
+        double *kj = kv + j * 3 * w;
+        for (int dd = 0; dd < 3; ++dd)
+            for (int m = 0; m < w; ++m) {
+                if (dd < dim) {
+                    const double t = d[3 * j + dd] + (double)m;
+                    const double a = 1.0 - t * t * ic;
+                    kj[w * dd + m] = a > 0.0 ? exp(beta * (sqrt(a) - 1.0)) : 0.0;
+                } else {
+                    kj[w * dd + m] = (m == 0) ? 1.0 : 0.0;
+                }
+            }
+    }
+    /* threads own contiguous slabs of the slowest grid dimension (z in 3D)
+     * and spread the part of every stencil that falls in their slab, so
+     * every grid cell is written by one thread in a fixed point order */
     const int sd = dim - 1;               /* slab dimension */
     const int64_t ns = nn[sd];
     /* bucket points by first kernel cell along the slab dimension */
@@ -527,7 +685,6 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
         for (int64_t j = 0; j < M; ++j) ord[fill[l[3 * j + sd]]++] = (int32_t)j;
         free(fill);
     }
-    const double hw = 0.5 * w, ic = 1.0 / (hw * hw);
 #pragma omp parallel
     {
         int nt = 1, tid = 0;
@@ -536,7 +693,6 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
         tid = omp_get_thread_num();
 #endif
         const int64_t s0 = ns * tid / nt, s1 = ns * (tid + 1) / nt;
-        double ker[3][16];
         if (s1 > s0) {
             /* first-cell values whose rows reach [s0, s1): l in [s0-w+1, s1-1];
              * when that range wraps onto itself every bucket is visited once */
@@ -547,16 +703,7 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
                 const int64_t lb = ((lv % ns) + ns) % ns;
                 for (int64_t q = start[lb]; q < start[lb + 1]; ++q) {
                     const int64_t j = ord[q];
-                    for (int dd = 0; dd < 3; ++dd)
-                        for (int m = 0; m < w; ++m) {
-                            if (dd < dim) {
-                                const double t = d[3 * j + dd] + (double)m;
-                                const double a = 1.0 - t * t * ic;
-                                ker[dd][m] = a > 0.0 ? exp(beta * (sqrt(a) - 1.0)) : 0.0;
-                            } else {
-                                ker[dd][m] = (m == 0) ? 1.0 : 0.0;
-                            }
-                        }
+                    const double *ker = kv + j * 3 * w;
                     const double cr = c[2 * j], ci = c[2 * j + 1];
                     const int w2 = dim >= 2 ? w : 1, w3 = dim >= 3 ? w : 1;
                     for (int m3 = 0; m3 < w3; ++m3) {
@@ -565,14 +712,23 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
                         for (int m2 = 0; m2 < w2; ++m2) {
                             const int64_t i2 = (l[3 * j + 1] + m2) % n2;
                             if (sd == 1 && (i2 < s0 || i2 >= s1)) continue;
-                            const double k23 = ker[1][m2] * ker[2][m3];
+                            const double k23 = ker[w + m2] * ker[2 * w + m3];
+                            const double sr = cr * k23, si = ci * k23;
                             double *row = grid + 2 * ((i3 * n2 + i2) * n1);
+                            const int64_t l1 = l[3 * j];
+                            if (sd != 0 && l1 + w <= n1) {
+                                double *rr = row + 2 * l1;
+                                for (int m1 = 0; m1 < w; ++m1) {
+                                    rr[2 * m1] += sr * ker[m1];
+                                    rr[2 * m1 + 1] += si * ker[m1];
+                                }
+                                continue;
+                            }
                             for (int m1 = 0; m1 < w; ++m1) {
-                                const int64_t i1 = (l[3 * j] + m1) % n1;
+                                const int64_t i1 = (l1 + m1) % n1;
                                 if (sd == 0 && (i1 < s0 || i1 >= s1)) continue;
-                                const double kv = ker[0][m1] * k23;
-                                row[2 * i1] += cr * kv;
-                                row[2 * i1 + 1] += ci * kv;
+                                row[2 * i1] += sr * ker[m1];
+                                row[2 * i1 + 1] += si * ker[m1];
                             }
                         }
                     }
@@ -580,6 +736,7 @@ int orc_es_spread(int dim, int64_t M, const double *x, const double *y,
             }
         }
     }
+    free(kv);
     free(l); free(d); free(key); free(start); free(ord);
     return 0;
 }

@@ -11,6 +11,7 @@ oracle.c for the reference lines each routine restates.
 from __future__ import annotations
 
 import ctypes as C
+import functools
 import os
 import subprocess
 
@@ -42,6 +43,10 @@ def lib():
         vp = C.c_void_p
         L.orc_dirft_type1.argtypes = [i32, i64, vp, vp, vp, vp, i32, i64, i64, i64, vp]
         L.orc_dirft_type1_sampled.argtypes = [i32, i64, vp, vp, vp, vp, i32, i64, vp, vp, vp, vp]
+        L.orc_dirft_type1_prod.argtypes = [i32, i64, vp, vp, vp, vp, i32, i64, vp, i64, vp, i64, vp, vp]
+        L.orc_qtables.argtypes = [i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp, vp, vp, vp, vp, vp,
+                                  i32, vp, vp, vp]
+        L.orc_qtables.restype = None
         L.orc_fold.argtypes = [i32, i64, vp, vp, vp, vp, i32, i32, i32, vp, vp, vp]
         L.orc_counting_sort.argtypes = [i64, vp, i32, vp, vp]
         L.orc_select_cell.argtypes = [i64, vp, vp, vp, vp, vp, vp]
@@ -118,6 +123,40 @@ def dirft_type1_sampled(x, y, z, c, k1, k2=None, k3=None, iflag=-1):
     if rc:
         raise RuntimeError(f"orc_dirft_type1_sampled rc={rc}")
     return f
+
+
+def dirft_type1_prod(x, y, z, c, k1s, k2s=None, k3s=None, iflag=-1):
+    """FP64 direct sum on the product set k1s x k2s x k3s (each at most 4096
+    values); returns complex128[len(k3s), len(k2s), len(k1s)]."""
+    dim = 1 + (k2s is not None) + (k3s is not None)
+    x, y, z = _coords(dim, x, y, z)
+    c = _c128(c)
+    k1s = np.ascontiguousarray(k1s, np.int64)
+    k2s = np.ascontiguousarray(k2s if k2s is not None else [0], np.int64)
+    k3s = np.ascontiguousarray(k3s if k3s is not None else [0], np.int64)
+    if max(k2s.size, k3s.size) > 4096:
+        raise ValueError("at most 4096 values of k2 and k3")
+    f = np.zeros((k3s.size, k2s.size, k1s.size), np.complex128)
+    rc = lib().orc_dirft_type1_prod(dim, x.size, _p(x), _p(y), _p(z), _p(c), iflag, k1s.size, _p(k1s),
+                                    k2s.size, _p(k2s), k3s.size, _p(k3s), _p(f))
+    if rc:
+        raise RuntimeError(f"orc_dirft_type1_prod rc={rc}")
+    return f
+
+
+def qtables(plane, x_ffpar=None):
+    """a8: (q2_norm [n_qx*n_qy], x_ffq [nsp, n_qx*n_qy] or None) of a Q plane
+    (the dict of mp_tools_amd.sqw.qplane), REAL(4) as mp_sqom56.f90:909-936."""
+    f32 = lambda v: np.ascontiguousarray(v, np.float32)
+    n_qx, n_qy = int(plane["n_qx"]), int(plane["n_qy"])
+    q2 = np.zeros(n_qx * n_qy, np.float32)
+    ff = None if x_ffpar is None else f32(x_ffpar).reshape(-1, 9)
+    nsp = 0 if ff is None else ff.shape[0]
+    xq = None if ff is None else np.zeros((nsp, n_qx * n_qy), np.float32)
+    vec = [f32(plane[k]) for k in ("e1", "e2", "q_v", "e1p", "e2p", "q_vp")]
+    lib().orc_qtables(n_qx, n_qy, float(plane["qx_range"][0]), float(plane["qy_range"][0]), float(plane["bz_nx"]),
+                      float(plane["bz_ny"]), *[_p(v) for v in vec], nsp, _p(ff), _p(q2), _p(xq))
+    return q2, xq
 
 
 def fold(x, y, z, n_fine, w, by=1, bz=1):
@@ -255,6 +294,13 @@ def es_phihat(k, n, w, beta, nodes: int = 200):
     return 2.0 * (np.cos(2.0 * np.pi * np.outer(k, t) / n) * (phi * wt)).sum(axis=1)
 
 
+@functools.lru_cache(maxsize=64)
+def _deconv(N, n, w, beta):
+    """1 / phi_hat on the kept modes k = -N/2 .. (N-1)/2 (cached per plan)."""
+    k = np.arange(-(N // 2), (N - 1) // 2 + 1)
+    return 1.0 / es_phihat(k, n, w, beta)
+
+
 def cpu_nufft_type1(x, y, z, c, n_modes, eps=1e-6, iflag=-1, workers=None):
     """Type-1 NUFFT by the FINUFFT method on the host: FP64 ES spreading
     (C, OpenMP), scipy.fft (pocketfft, multi-threaded), deconvolution.
@@ -285,8 +331,7 @@ def cpu_nufft_type1(x, y, z, c, n_modes, eps=1e-6, iflag=-1, workers=None):
         k = np.arange(-(N // 2), (N - 1) // 2 + 1)
         idx = np.mod(k, n)
         out = np.take(out, idx, axis=ax_rev)
-        ph = es_phihat(k, n, w, beta)
         shape = [1] * dim
         shape[ax_rev] = N
-        out = out / ph.reshape(shape)
+        out = out * _deconv(N, n, w, beta).reshape(shape)
     return out

@@ -6,9 +6,10 @@ C1: one static frame of a 20^3 perovskite (40,000 atoms, 3 species, neutron
     atoms (the definition of dirft2d1, code/src/mp_sqom56.f90:2530-2602) at
     2,000 seeded modes.
 C3: 128^3 perovskite (10,485,760 atoms), 512x512x64 modes, eps in
-    {1e-3, 1e-4, 1e-6} (FP32 and FP64 forms) and 1e-9 (FP64), checked
-    against the FP64 direct DFT at seeded random modes (120 here to keep the
-    CPU oracle to seconds; bench-scale runs use more).
+    {1e-3, 1e-4, 1e-6, 1e-9}, FP32 and FP64 forms (FP32 floors eps at 1e-7),
+    checked against the FP64 direct DFT at 10,000 seeded random modes (the
+    product set 25 x 25 x 16 of tests/golden/c3_dft_10k.npz, written by
+    tests/golden/make_c3_dft.py from the same seeded frame).
 
 Tolerances (relative l2 over the sampled modes): FP64 form 10 * eps;
 FP32 form max(10 * eps, 5e-5).  C1 is a nearly perfect lattice: its
@@ -87,35 +88,42 @@ def test_c1_static_elastic_map(gpu_lib, eps):
 
 
 @pytest.mark.parametrize("precision,eps", [("fp64", 1e-9), ("fp64", 1e-6), ("fp64", 1e-4), ("fp64", 1e-3),
-                                           ("fp32", 1e-6), ("fp32", 1e-4), ("fp32", 1e-3)])
+                                           ("fp32", 1e-9), ("fp32", 1e-6), ("fp32", 1e-4), ("fp32", 1e-3)])
 def test_c3_precision_sweep(gpu_lib, precision, eps, c3_frame):
     import torch
     from mp_tools_amd.sqw import SqwEngine
-    pos, counts, ncell, w, dpos, ref, modes = c3_frame
+    counts, ncell, w, dpos, F, sets = c3_frame
     N = (512, 512, 64)
     eng = SqwEngine(3, N, counts, eps=eps, precision=precision, weights=w,
                     tpe=(2 * np.pi / ncell,) * 3)
     amp = eng.amplitudes(dpos)
     torch.cuda.synchronize()
-    got = _pick(amp.cpu().numpy()[0], N, *modes)
-    err = rel_l2(got, ref)
-    print(f"C3 {precision} eps={eps:g}: rel l2 vs direct DFT at {len(ref)} modes = {err:.3e}")
+    idx = torch.tensor(_flat_index(N, *sets), device="cuda")
+    got = amp[0, idx].cpu().numpy()
+    err = rel_l2(got, F.ravel())
+    print(f"C3 {precision} eps={eps:g}: rel l2 vs direct DFT at {F.size} modes = {err:.3e}")
     tol = 10 * eps if precision == "fp64" else max(10 * eps, 5e-5)
     assert err < tol
     del amp, eng
     torch.cuda.empty_cache()
 
 
+def _flat_index(N, k1s, k2s, k3s):
+    """CMCL flat indices of the product set, in its [k3][k2][k1] order."""
+    K3, K2, K1 = np.meshgrid(k3s, k2s, k1s, indexing="ij")
+    return ((K3 + N[2] // 2) * N[1] * N[0] + (K2 + N[1] // 2) * N[0] + (K1 + N[0] // 2)).ravel()
+
+
 @pytest.fixture(scope="module")
 def c3_frame():
+    import os
     import torch
     from mp_tools_amd import synth
-    ncell = 128
-    pos, counts = synth.static_frame(ncell, sigma=0.05, seed=20261205)
-    w = 0.1 * synth.B_NEUTRON_FM
-    rng = np.random.default_rng(3)
-    modes = _modes(rng, (512, 512, 64), 120)
-    x, y, z, c = _staged(pos, counts, ncell, w)
-    ref = O.dirft_type1_sampled(x, y, z, c, *modes)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c3_dft_10k.npz"))
+    ncell = int(g["ncell"])
+    pos, counts = synth.static_frame(ncell, sigma=0.05, seed=int(g["seed"]))
+    p = pos[0, :, :3].astype(np.float64)
+    assert np.allclose([p.sum(), (p * p).sum(), p[::9973].sum()], g["checksums"], rtol=1e-12), \
+        "the seeded frame differs from the one the fixture was made from"
     dpos = torch.tensor(pos, device="cuda")
-    return pos, counts, ncell, w, dpos, ref, modes
+    return counts, ncell, g["weights"], dpos, g["F"], (g["k1s"], g["k2s"], g["k3s"])
