@@ -176,6 +176,17 @@ typedef struct mpsq_reduce_spec {
     int64_t n_f;              /* number of frequency bins written */
     int subtract_mean;        /* 1: inelastic C' = C - <C>_t (:1358-1377) */
     const float *d_qfac;      /* device [NQ] factor on C' (q2_norm when j_qsq != 1), or NULL */
+    int kind;                 /* 0: S(Q,w), frequencies m_lo .. m_lo+n_f-1 (E = const maps :1437-1478 and
+                                 E(Q) cuts :1480-1517); 1, 2: the intermediate function I(Q,t) (j_sq != 1),
+                                 d_S[l][Q] = Re sum_{i<n_int} C'(i+lag, Q) conj C'(i, Q) for lag = m_lo + l,
+                                 0 <= lag <= T - n_int, times 1/n_int for kind 1 (the map, :1467-1471) and
+                                 unscaled for kind 2 (the cut, :1504-1510), over nat^2 (:1574-1575) */
+    int odd_n_int;            /* 1: the reference's rule (:683-686): n_int = 0 means T/2, and for S(Q,w) an
+                                 even n_int grows by one */
+    const int32_t *d_qidx;    /* device [n_qidx] mode indices to reduce (the pixels of an E(Q) cut), or NULL
+                                 for every mode; d_S is then [n_f][n_qidx], d_Stot [n_qidx], d_qfac per mode;
+                                 an index outside [0, NQ) gives NaN outputs in its column */
+    int64_t n_qidx;
 } mpsq_reduce_spec;
 
 /* The reference's per-Q tables of a Q plane (mp_sqom56.f90:909-936),
@@ -204,6 +215,19 @@ typedef struct mpsq_qplane {
 } mpsq_qplane;
 int mpsq_qtables(const mpsq_qplane *qp, int nsp, const float *x_ffpar, float *d_q2_norm, float *d_x_ffq,
                  void *stream);
+
+/* The speckle filter (mp_sqom56.f90:1530-1570) on a map in host memory,
+ * in place: one sequential pass over the n_rows x n_cols row-major array,
+ * walking it row-major (column_walk 0) or column-major (1), the neighbours
+ * of element k being k - 1 and k + 1 (cyclic) as the walk has left them: a
+ * value below min(neighbours) / s_trig becomes that minimum, a value above
+ * s_trig * max(neighbours) that maximum.  s_trig <= 0 leaves the map as it
+ * is.  The reference filters an E = const map twice (x fastest, then y
+ * fastest: column_walk 0 then 1 on a [n_qy][n_qx] map) and an E(Q) cut of
+ * all n_freq_max frequencies once, frequency fastest (column_walk 1 on
+ * d_S[n_f][n_q]); it does so before the 1/nat^2 norm, which does not change
+ * the filter's decisions. */
+int mpsq_speckle_filter(double *map, int64_t n_rows, int64_t n_cols, int column_walk, double s_trig);
 
 /* d_S: device double[n_f][NQ]; d_Stot: device double[NQ] or NULL. */
 int mpsq_sqw_reduce(mpsq_sqw eng, const void *d_amp, int64_t T, const mpsq_reduce_spec *r, double *d_S,

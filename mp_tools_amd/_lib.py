@@ -36,6 +36,7 @@ EXPORTS = [
     "mpsq_sqw_default_spec", "mpsq_sqw_create", "mpsq_sqw_amplitudes", "mpsq_sqw_reduce",
     "mpsq_sqw_destroy", "mpsq_sqw_sync", "mpsq_sqw_nq", "mpsq_sqw_nact", "mpsq_version",
     "mpsq_sqw_set_timing", "mpsq_sqw_get_timing", "mpsq_sqw_debug_sorted", "mpsq_sqw_info", "mpsq_qtables",
+    "mpsq_speckle_filter",
     "mpsq_batch_create", "mpsq_batch_run", "mpsq_batch_set_budget", "mpsq_batch_set_timing",
     "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
     "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
@@ -92,7 +93,8 @@ class Request(C.Structure):
 class ReduceSpec(C.Structure):
     _fields_ = [("n_int", C.c_int64), ("win_kind", C.c_int), ("dt", C.c_double),
                 ("nat", C.c_double), ("m_lo", C.c_int64), ("n_f", C.c_int64),
-                ("subtract_mean", C.c_int), ("d_qfac", C.c_void_p)]
+                ("subtract_mean", C.c_int), ("d_qfac", C.c_void_p), ("kind", C.c_int), ("odd_n_int", C.c_int),
+                ("d_qidx", C.c_void_p), ("n_qidx", C.c_int64)]
 
 
 _lib = None
@@ -151,6 +153,7 @@ def load():
     L.mpsq_sqw_set_timing.argtypes = [vp, i32]
     L.mpsq_sqw_debug_sorted.argtypes = [vp, vp, i64, vp, vp, vp, i64]
     L.mpsq_qtables.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.mpsq_speckle_filter.argtypes = [vp, i64, i64, i32, C.c_double]
     L.mpsq_sqw_info.argtypes = [vp, vp]
     L.mpsq_sqw_get_timing.argtypes = [vp, vp, vp, i32]
     L.mpsq_batch_create.argtypes = [i32, vp, vp]

@@ -1117,21 +1117,43 @@ template <typename R> struct Engine {
     {
         Core<R> &c = core;
         if (!combine || !d_amp || !r || (!d_S && r->n_f > 0) || T < 1) throw Err{MPSQ_ERR_BAD_ARG};
-        if (r->n_int < 1 || r->n_int > T || r->n_f < 0 || !(r->nat > 0)) throw Err{MPSQ_ERR_BAD_ARG};
+        if (r->kind < 0 || r->kind > 2 || (r->d_qidx && r->n_qidx < 1)) throw Err{MPSQ_ERR_BAD_ARG};
+        // the reference's window length rule (mp_sqom56.f90:683-686)
+        int64_t n_int = r->n_int;
+        if (r->odd_n_int) {
+            if (n_int == 0) n_int = T / 2;
+            if (r->kind == 0 && n_int % 2 == 0) n_int += 1;
+        }
+        if (n_int < 1 || n_int > T || r->n_f < 0 || !(r->nat > 0)) throw Err{MPSQ_ERR_BAD_ARG};
+        if (r->kind != 0 && (r->m_lo < 0 || r->m_lo + r->n_f - 1 > T - n_int)) throw Err{MPSQ_ERR_BAD_ARG};
         HIPCHK(hipSetDevice(c.device));
         const int64_t NQ = c.N[0] * c.N[1] * c.N[2];
         SqwParams p{};
         p.T = T;
-        p.NQ = NQ;
-        p.n_int = (int)r->n_int;
-        p.n_frame = T - r->n_int + 1;
+        p.NQs = NQ;
+        p.NQ = r->d_qidx ? r->n_qidx : NQ;
+        p.qidx = r->d_qidx;
+        p.kind = r->kind;
+        p.n_int = (int)n_int;
+        p.n_frame = T - n_int + 1;
         p.m_lo = r->m_lo;
         p.n_f = (int)r->n_f;
         p.win_kind = r->win_kind;
         p.subtract_mean = r->subtract_mean;
         p.norm_s = r->dt / (r->nat * r->nat * 0.5 * (double)p.n_frame * (double)p.n_int);
+        if (p.kind != 0) p.norm_s = 1.0 / (r->nat * r->nat * (p.kind == 1 ? (double)p.n_int : 1.0));
         p.norm_tot = 1.0 / ((double)T * r->nat * r->nat);
         const Cx<R> *amp = static_cast<const Cx<R> *>(d_amp);
+        if (p.kind != 0) {
+            const int ev = tm.begin(ST_REDUCE, st);
+            launch_iqt<R>(p, amp, r->d_qfac, d_S, d_Stot, st);
+            tm.end(ev, st);
+            HIPCHK(hipGetLastError());
+            if (!sync) return;
+            HIPCHK(hipStreamSynchronize(st));
+            tm.collect();
+            return;
+        }
         RadixPlan rp{};
         const bool single = (p.n_frame == 1) && (T <= sqw_tile_elems<R>()) &&
                             sqw_single_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
@@ -1568,6 +1590,10 @@ template <typename R> struct Batch {
                 key.push_back((uint64_t)r[j].n_f);
                 key.push_back((uint64_t)r[j].subtract_mean);
                 kp(r[j].d_qfac);
+                key.push_back((uint64_t)r[j].kind);
+                key.push_back((uint64_t)r[j].odd_n_int);
+                kp(r[j].d_qidx);
+                key.push_back((uint64_t)r[j].n_qidx);
             }
         }
         if (use_graph && gexec && key == gkey) {
@@ -2324,6 +2350,25 @@ int mpsq_qtables(const mpsq_qplane *qp, int nsp, const float *x_ffpar, float *d_
     return hipGetLastError() == hipSuccess ? 0 : MPSQ_ERR_DEVICE;
 }
 
+// The reference's speckle filter (mp_sqom56.f90:1530-1570), host memory:
+// one sequential pass over the n_rows x n_cols row-major array in the walk
+// order (row-major, or column-major when column_walk), cyclic neighbours,
+// each value compared with its neighbours as the walk has left them.
+int mpsq_speckle_filter(double *map, int64_t n_rows, int64_t n_cols, int column_walk, double s_trig)
+{
+    if (!map || n_rows < 1 || n_cols < 1) return MPSQ_ERR_BAD_ARG;
+    if (!(s_trig > 0.0)) return 0;  // s_trig = 0 turns the filter off (:1531)
+    const int64_t n = n_rows * n_cols;
+    auto at = [&](int64_t k) -> double & { return column_walk ? map[(k % n_rows) * n_cols + k / n_rows] : map[k]; };
+    for (int64_t k = 0; k < n; ++k) {
+        const double a = at(k == 0 ? n - 1 : k - 1), b = at(k + 1 == n ? 0 : k + 1);
+        double &v = at(k);
+        if (v < std::min(a, b) / s_trig) v = std::min(a, b);
+        if (v > s_trig * std::max(a, b)) v = std::max(a, b);
+    }
+    return 0;
+}
+
 int mpsq_sqw_sync(mpsq_sqw e)
 {
     if (!e) return MPSQ_ERR_BAD_ARG;
@@ -2397,7 +2442,7 @@ int mpsq_batch_run(mpsq_batch b, const float *const *d_pos, const int64_t *T, vo
         const size_t nj = b->bf ? b->bf->jobs.size() : b->bd->jobs.size();
         for (size_t j = 0; j < nj; ++j) {
             if (!d_pos[j] || !d_amp[j] || T[j] < 1) throw Err{MPSQ_ERR_BAD_ARG};
-            if (r && (r[j].n_int < 1 || r[j].n_int > T[j] || r[j].n_f < 0 || !(r[j].nat > 0) ||
+            if (r && (r[j].n_int < (r[j].odd_n_int ? 0 : 1) || r[j].n_int > T[j] || r[j].n_f < 0 || !(r[j].nat > 0) ||
                       (r[j].n_f > 0 && (!d_S || !d_S[j]))))
                 throw Err{MPSQ_ERR_BAD_ARG};
         }

@@ -147,14 +147,17 @@ template <typename R> struct ColJob {
 constexpr int SQW_MAX_COLS = 64;
 
 struct SqwParams {
-    int64_t T, NQ;
+    int64_t T, NQ;      // frames; modes reduced (outputs [n_f][NQ], [NQ])
+    int64_t NQs;        // modes per frame of the amplitudes (row stride)
+    const int *qidx;    // mode of output q (a cut), or nullptr: q itself
+    int kind;           // 0 S(Q,w); 1, 2 I(Q,t) (1: times 1/n_int)
     int n_int;
     int64_t n_frame;
     int64_t m_lo;
     int n_f;
     int win_kind;       // 0: 0.5(1-cos(2pi i/(n-1))), 1: 0.5(1-cos(2pi(i+1)/(n+1)))
     int subtract_mean;
-    double norm_s;      // dt / (nat^2 * 0.5 * n_frame * n_int)
+    double norm_s;      // S(Q,w): dt / (nat^2 * 0.5 * n_frame * n_int); I(Q,t): 1 / (nat^2 [* n_int])
     double norm_tot;    // 1 / (T * nat^2)
 };
 
@@ -253,5 +256,9 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 template <typename R>
 void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S,
                         double *Stot, hipStream_t st);
+// I(Q,t): S[l][q] = norm Re sum_{i<n_int} C'(i + m_lo + l) conj C'(i), and S_tot
+template <typename R>
+void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,
+                hipStream_t st);
 
 }  // namespace mpsq

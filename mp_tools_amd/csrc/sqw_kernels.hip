@@ -17,6 +17,16 @@
 
 namespace mpsq {
 
+// amplitude column of output q; -1 for an index list entry outside
+// [0, NQs), whose outputs are written as NaN (nothing is read for it)
+__device__ __forceinline__ int64_t src_q(const SqwParams &p, int64_t q)
+{
+    if (!p.qidx) return q;
+    const int64_t s = p.qidx[q];
+    return (s >= 0 && s < p.NQs) ? s : -1;
+}
+__device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000000000000LL); }
+
 template <typename R, int E, int NT>
 __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, int cols, FDiv dcols, FDiv dT,
                                                    const typename cplx<R>::T *__restrict__ amp,
@@ -41,7 +51,8 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         for (int k = 0; k < E; ++k) {
             const int i = threadIdx.x + k * NT;
             const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
-            v[k] = (i < T * cols && c < ncol) ? amp[(int64_t)t * p.NQ + q0 + c] : cmk<C>((R)0, (R)0);
+            const int64_t s = (i < T * cols && c < ncol) ? src_q(p, q0 + c) : -1;
+            v[k] = (s >= 0) ? amp[(int64_t)t * p.NQs + s] : cmk<C>((R)0, (R)0);
         }
 #pragma unroll
         for (int k = 0; k < E; ++k) {
@@ -83,7 +94,8 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
             }
             mean_r[threadIdx.x] = p.subtract_mean ? a / (double)T : 0.0;
             mean_i[threadIdx.x] = p.subtract_mean ? b / (double)T : 0.0;
-            if (Stot && threadIdx.x < ncol) Stot[q0 + threadIdx.x] = q * p.norm_tot;
+            if (Stot && threadIdx.x < ncol)
+                Stot[q0 + threadIdx.x] = src_q(p, q0 + threadIdx.x) >= 0 ? q * p.norm_tot : qnan();
         }
         __syncthreads();
     }
@@ -91,7 +103,10 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
         const C v = tile[i];
         double f = win[t];
-        if (qfac && c < ncol) f *= (double)qfac[q0 + c];
+        if (qfac && c < ncol) {
+            const int64_t s = src_q(p, q0 + c);
+            if (s >= 0) f *= (double)qfac[s];
+        }
         tile[i] = cmk<C>((R)(((double)v.x - mean_r[c]) * f), (R)(((double)v.y - mean_i[c]) * f));
     }
     __syncthreads();
@@ -104,7 +119,9 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         const int mm = m0 + mi;
         const int m = mm - (int)fdiv((uint32_t)mm, dT) * T;
         const C v = tile[m * cols + c];
-        S[(int64_t)mi * p.NQ + q0 + c] = ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * p.norm_s;
+        S[(int64_t)mi * p.NQ + q0 + c] = src_q(p, q0 + c) >= 0
+                                             ? ((double)v.x * (double)v.x + (double)v.y * (double)v.y) * p.norm_s
+                                             : qnan();
     }
 }
 
@@ -119,9 +136,17 @@ __global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename
     if (q >= p.NQ || grp * MG >= p.n_f) return;
     const int64_t T = p.T;
     const int n = p.n_int;
+    const int64_t qs = src_q(p, q);
+    if (qs < 0) {
+        if (grp == 0 && Stot) Stot[q] = qnan();
+        for (int j = 0; j < MG; ++j)
+            if (grp * MG + j < p.n_f) S[(int64_t)(grp * MG + j) * p.NQ + q] = qnan();
+        return;
+    }
+    amp += qs;
     double mr = 0.0, mi = 0.0, ss = 0.0;
     for (int64_t t = 0; t < T; ++t) {
-        const auto v = amp[t * p.NQ + q];
+        const auto v = amp[t * p.NQs];
         mr += (double)v.x;
         mi += (double)v.y;
         ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
@@ -133,7 +158,7 @@ __global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename
     } else {
         mr = mi = 0.0;
     }
-    const double qf = qfac ? (double)qfac[q] : 1.0;
+    const double qf = qfac ? (double)qfac[qs] : 1.0;
     const double twopi = 6.283185307179586476925286766559;
     // window = a0 + a1 e^{i th i} + a2 e^{-i th i}
     double th = 0.0, a0 = 1.0, a1r = 0.0, a1i = 0.0;
@@ -166,7 +191,7 @@ __global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename
         acc[j] = 0.0;
     }
     for (int i = 0; i < n; ++i) {
-        const auto v = amp[(int64_t)i * p.NQ + q];
+        const auto v = amp[(int64_t)i * p.NQs];
         const double cr = ((double)v.x - mr) * qf, ci = ((double)v.y - mi) * qf;
 #pragma unroll
         for (int j = 0; j < MG; ++j)
@@ -190,8 +215,8 @@ __global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename
             acc[j] += xr * xr + xi * xi;
         }
         if (s + 1 >= p.n_frame) break;
-        const auto vo = amp[s * p.NQ + q];
-        const auto vn = amp[(s + n) * p.NQ + q];
+        const auto vo = amp[s * p.NQs];
+        const auto vn = amp[(s + n) * p.NQs];
         const double or_ = ((double)vo.x - mr) * qf, oi = ((double)vo.y - mi) * qf;
         const double nr_ = ((double)vn.x - mr) * qf, ni = ((double)vn.y - mi) * qf;
 #pragma unroll
@@ -301,6 +326,78 @@ void launch_qtables(const QTableParams &p, float *q2_norm, float *x_ffq, hipStre
     hipLaunchKernelGGL(k_qtables, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, p, q2_norm, x_ffq);
 }
 
+// I(Q,t) (mp_sqom56.f90:1467-1471 map, :1504-1510 cut): thread = (output q,
+// group of MG lags); C' = (C - <C>_t) * qfac as in the S(Q,w) kernels; the
+// products and sums in FP64, the real part written times norm_s (1/nat^2,
+// and 1/n_int for the map form)
+template <typename R, int MG>
+__global__ __launch_bounds__(256) void k_iqt(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
+                                             const float *__restrict__ qfac, double *__restrict__ S,
+                                             double *__restrict__ Stot)
+{
+    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int grp = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (q >= p.NQ || grp * MG >= p.n_f) return;
+    const int64_t qs = src_q(p, q);
+    if (qs < 0) {
+        if (grp == 0 && Stot) Stot[q] = qnan();
+        for (int j = 0; j < MG; ++j)
+            if (grp * MG + j < p.n_f) S[(int64_t)(grp * MG + j) * p.NQ + q] = qnan();
+        return;
+    }
+    amp += qs;
+    double mr = 0.0, mi = 0.0, ss = 0.0;
+    for (int64_t t = 0; t < p.T; ++t) {
+        const auto v = amp[t * p.NQs];
+        mr += (double)v.x;
+        mi += (double)v.y;
+        ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    if (grp == 0 && Stot) Stot[q] = ss * p.norm_tot;
+    if (p.subtract_mean) {
+        mr /= (double)p.T;
+        mi /= (double)p.T;
+    } else {
+        mr = mi = 0.0;
+    }
+    const double qf = qfac ? (double)qfac[qs] : 1.0;
+    double acc[MG];
+    int64_t lag[MG];
+#pragma unroll
+    for (int j = 0; j < MG; ++j) {
+        acc[j] = 0.0;
+        lag[j] = p.m_lo + grp * MG + j;
+    }
+    for (int i = 0; i < p.n_int; ++i) {
+        const auto a = amp[(int64_t)i * p.NQs];
+        const double ar = ((double)a.x - mr) * qf, ai = ((double)a.y - mi) * qf;
+#pragma unroll
+        for (int j = 0; j < MG; ++j) {
+            const int64_t t = i + lag[j];
+            if (grp * MG + j < p.n_f && t >= 0 && t < p.T) {
+                const auto b = amp[t * p.NQs];
+                const double br = ((double)b.x - mr) * qf, bi = ((double)b.y - mi) * qf;
+                acc[j] += br * ar + bi * ai;  // Re(b conj(a))
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < MG; ++j) {
+        const int l = grp * MG + j;
+        if (l < p.n_f) S[(int64_t)l * p.NQ + q] = acc[j] * p.norm_s;
+    }
+}
+
+template <typename R>
+void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,
+                hipStream_t st)
+{
+    constexpr int MG = 4;
+    const unsigned gx = (unsigned)((p.NQ + 63) / 64);
+    const unsigned gy = (unsigned)((p.n_f + 4 * MG - 1) / (4 * MG));
+    hipLaunchKernelGGL((k_iqt<R, MG>), dim3(gx, gy > 0 ? gy : 1), dim3(256), 0, st, p, amp, qfac, S, Stot);
+}
+
 template <typename R> struct SqwTiles;
 template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
 template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };
@@ -348,7 +445,8 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
                                        const cplx<R>::T *, const double *, const float *, double *, double *,        \
                                        hipStream_t);                                                                  \
     template void launch_sqw_sliding<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *,    \
-                                        hipStream_t);
+                                        hipStream_t);                                                                 \
+    template void launch_iqt<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *, hipStream_t);
 
 MPSQ_SQW_INST(float)
 MPSQ_SQW_INST(double)

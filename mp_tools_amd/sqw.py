@@ -140,23 +140,42 @@ class SqwEngine:
         return out
 
     def reduce(self, amp, n_int=None, win_kind=0, dt=1.0, nat=1.0, m_lo=0, n_f=None,
-               subtract_mean=True, qfac=None, out=None, out_tot=None):
-        """S(Q,w) [n_f, NQ] and S_tot(Q) [NQ] (float64 CUDA tensors)."""
+               subtract_mean=True, qfac=None, out=None, out_tot=None, kind=0, odd_n_int=False, qidx=None):
+        """S(Q,w) [n_f, NQ] and S_tot(Q) [NQ] (float64 CUDA tensors).
+
+        kind 0: S(Q,w) at frequencies m_lo .. m_lo+n_f-1; kind 1 / 2: the
+        intermediate function I(Q,t) at lags m_lo .. (map form, 1/n_int;
+        cut form, unscaled).  odd_n_int: the reference's window length rule
+        (n_int 0 = T/2; S(Q,w) takes an odd n_int).  qidx: mode indices to
+        reduce (an E(Q) cut); outputs are then [n_f, len(qidx)]."""
         import torch
 
         T = amp.shape[0]
-        n_int = T if n_int is None else int(n_int)
-        n_f = n_int if n_f is None else int(n_f)
+        if odd_n_int:
+            n_int = 0 if n_int is None else int(n_int)
+            _, n_all = reference_n_int(T, n_int, sqw=(int(kind) == 0))
+        else:
+            n_int = T if n_int is None else int(n_int)
+            n_all = n_int if int(kind) == 0 else T - n_int + 1
+        n_f = n_all if n_f is None else int(n_f)
         r = _lib.ReduceSpec()
         r.n_int, r.win_kind, r.dt, r.nat = n_int, int(win_kind), float(dt), float(nat)
         r.m_lo, r.n_f, r.subtract_mean = int(m_lo), n_f, int(bool(subtract_mean))
+        r.kind, r.odd_n_int = int(kind), int(bool(odd_n_int))
+        nq = self.NQ
         if qfac is not None:
             qf = qfac.to(torch.float32).contiguous()
             r.d_qfac = qf.data_ptr()
+        if qidx is not None:
+            qi = torch.as_tensor(qidx, dtype=torch.int32, device=amp.device).contiguous()
+            if qi.numel() < 1 or int(qi.min()) < 0 or int(qi.max()) >= self.NQ:
+                raise ValueError(f"qidx entries must lie in [0, {self.NQ})")
+            r.d_qidx, r.n_qidx = qi.data_ptr(), qi.numel()
+            nq = qi.numel()
         if out is None:
-            out = torch.empty((n_f, self.NQ), dtype=torch.float64, device=amp.device)
+            out = torch.empty((n_f, nq), dtype=torch.float64, device=amp.device)
         if out_tot is None:
-            out_tot = torch.empty(self.NQ, dtype=torch.float64, device=amp.device)
+            out_tot = torch.empty(nq, dtype=torch.float64, device=amp.device)
         rc = self._L.mpsq_sqw_reduce(self._h, amp.data_ptr(), T, C.byref(r), out.data_ptr(),
                                      out_tot.data_ptr())
         _lib.check("sqw_reduce", rc)
@@ -209,10 +228,12 @@ class SqwEngine:
             pass
 
 
-def _reduce_spec(n_int, win_kind=0, dt=1.0, nat=1.0, m_lo=0, n_f=None, subtract_mean=True, qfac=None):
+def _reduce_spec(n_int, win_kind=0, dt=1.0, nat=1.0, m_lo=0, n_f=None, subtract_mean=True, qfac=None, kind=0,
+                 odd_n_int=False):
     r = _lib.ReduceSpec()
     r.n_int, r.win_kind, r.dt, r.nat = int(n_int), int(win_kind), float(dt), float(nat)
     r.m_lo, r.n_f, r.subtract_mean = int(m_lo), int(n_int if n_f is None else n_f), int(bool(subtract_mean))
+    r.kind, r.odd_n_int = int(kind), int(bool(odd_n_int))
     if qfac is not None:
         r.d_qfac = qfac.data_ptr()
     return r
@@ -379,7 +400,7 @@ def qplane(n_row, bz_n: float, e1=(1, 0, 0), e2=(0, 1, 0), q_center=(0.0, 0.0, 0
     q_vp = (f(np.dot(evp, qc) / evpn) * evp).astype(f) if evpn > 0 else np.zeros(3, f)
     return dict(e1=e1, e2=e2, ev=ev, tpe=(tpe1, tpe2, tpev), bz_nx=bz_nx, bz_ny=bz_ny, n_qx=n_qx, n_qy=n_qy,
                 qx_range=(qx_min, qx_max), qy_range=(qy_min, qy_max), n_modes=(n_qx, n_qy),
-                e1p=e1p, e2p=e2p, evp=evp, q_v=q_v, q_vp=q_vp,
+                e1p=e1p, e2p=e2p, evp=evp, q_v=q_v, q_vp=q_vp, e_norms=(n1, n2, nv),
                 engine_kw=dict(axes=np.stack([e1, e2, ev]).astype(np.float64),
                                tpe=tuple(float(t) for t in (tpe1, tpe2, tpev)), q0=tuple(float(v) for v in qc)))
 
@@ -415,3 +436,120 @@ def qtables(plane, x_ffpar=None, stream=None, device=0):
                         None if xq is None else xq.data_ptr(), stream)
     _lib.check("qtables", rc)
     return q2, xq
+
+
+def _nint(x):
+    """Fortran NINT: nearest integer, halves away from zero."""
+    x = float(x)
+    return int(np.floor(x + 0.5)) if x >= 0 else -int(np.floor(-x + 0.5))
+
+
+def reference_n_int(nfile, n_int=0, sqw=True):
+    """The reference's window length (mp_sqom56.f90:683-701): n_int = 0 means
+    nfile / 2; for S(Q,w) an even n_int grows by one.  Returns (n_int,
+    n_freq_max): (n_int - 1) / 2 + 1 frequencies of S(Q,w), or the
+    nfile - n_int + 1 lags of I(Q,t)."""
+    n_int = int(n_int) if n_int else int(nfile) // 2
+    if sqw:
+        if n_int % 2 == 0:
+            n_int += 1
+        return n_int, (n_int - 1) // 2 + 1
+    return n_int, int(nfile) - n_int + 1
+
+
+def econst_stack(f_map_min, f_map_step, n_plot, freq_step, iqt=False):
+    """Indices of the maps of an E = const (or t = const) stack
+    (mp_sqom56.f90:1266-1284, 1437-1438): plots i_plot = 1, 1 + i_step, ...
+    <= n_plot with n_freq_min = nint(f_map_min / freq_step) + 1, i_step =
+    nint(f_map_step / freq_step) (a smaller step is raised to freq_step, as
+    the reference's dialogue does) and j_freq = n_freq_min + i_plot - 1.
+    S(Q,w): frequency m = j_freq - 1 of exp(-2 pi i m t / n_int) (:1443);
+    I(Q,t) (freq_step = the frame step, :701): the map correlates frames
+    j_freq + 1 .. j_freq + n_int with 1 .. n_int (:1468-1469), i.e. lag
+    j_freq.  Returns (indices for SqwEngine.reduce, plot labels
+    f_map_min + (i_plot - 1) freq_step)."""
+    f = np.float32
+    freq_step = f(freq_step)
+    n_freq_min = _nint(f(f_map_min) / freq_step) + 1
+    i_step = _nint(f(f_map_step) / freq_step)
+    if i_step < 1:
+        i_step = _nint(freq_step / freq_step)
+    plots = list(range(1, int(n_plot) + 1, i_step))
+    j_freq = [n_freq_min + i - 1 for i in plots]
+    idx = j_freq if iqt else [j - 1 for j in j_freq]
+    labels = [float(f(f_map_min) + f(i - 1) * freq_step) for i in plots]
+    return idx, labels
+
+
+def qcut(plane, q1_3d, q2_3d, n_row, i_plot=1, n_plot=1, dq_p=(0.0, 0.0, 0.0)):
+    """Pixels of an E(Q) / I(Q,t) cut (mp_sqom56.f90:1290-1301, 1389-1432,
+    1489-1498) in the reference's REAL(4) arithmetic: Q1, Q2 (r.l.u.) minus
+    the plane's vertical component, the in-plane test, the plot i_plot of a
+    series of n_plot shifted by dq_p, the end pixels n_q1x = nint(|q1_x -
+    qx_min| (e1 . n_row)) + 1 (and y), n_qdisp steps along the longer axis,
+    and pixel ii = (n_q1x + nint((ii-1) q_nx), n_q1y + nint((ii-1) q_ny)) for
+    ii = 1 .. n_qdisp.  Returns a dict with the CMCL mode index of every
+    pixel (``qidx``, for SqwEngine.reduce) and the axis (q_min, q_max, label).
+    Raises ValueError where the reference prints and asks again."""
+    f = np.float32
+    e1, e2, ev = (np.asarray(plane[k], f) for k in ("e1", "e2", "ev"))
+    q_v = np.asarray(plane["q_v"], f)
+    q1 = (np.asarray(q1_3d, f) - q_v).astype(f)
+    q2 = (np.asarray(q2_3d, f) - q_v).astype(f)
+    if f(np.dot(q1, ev)) != 0 or f(np.dot(q2, ev)) != 0:
+        raise ValueError("Q1 or Q2 is not in the Q plane")
+    e1n, e2n = f(plane["e_norms"][0]), f(plane["e_norms"][1])
+    sh = (f(i_plot - int(n_plot) // 2 - 1) * np.asarray(dq_p, f)).astype(f)
+    q1_x, q1_y = f(np.dot(e1, (q1 + sh).astype(f)) / e1n), f(np.dot(e2, (q1 + sh).astype(f)) / e2n)
+    q2_x, q2_y = f(np.dot(e1, (q2 + sh).astype(f)) / e1n), f(np.dot(e2, (q2 + sh).astype(f)) / e2n)
+    (qx_min, qx_max), (qy_min, qy_max) = plane["qx_range"], plane["qy_range"]
+    for qx, qy in ((q1_x, q1_y), (q2_x, q2_y)):
+        if qx < qx_min or qx > qx_max or qy < qy_min or qy > qy_max:
+            raise ValueError("Q1 or Q2 is out of the map range")
+    nr = np.asarray(n_row, f)
+    e1r, e2r = f(np.dot(e1, nr)), f(np.dot(e2, nr))
+    n_q1x, n_q2x = _nint(f(abs(f(q1_x - qx_min)) * e1r)) + 1, _nint(f(abs(f(q2_x - qx_min)) * e1r)) + 1
+    n_q1y, n_q2y = _nint(f(abs(f(q1_y - qy_min)) * e2r)) + 1, _nint(f(abs(f(q2_y - qy_min)) * e2r)) + 1
+    if abs(n_q2x - n_q1x) >= abs(n_q2y - n_q1y):
+        n_qdisp, q_min, q_max, label = abs(n_q2x - n_q1x), q1_x, q2_x, "Q_x"
+    else:
+        n_qdisp, q_min, q_max, label = abs(n_q2y - n_q1y), q1_y, q2_y, "Q_y"
+    if n_qdisp == 0:
+        raise ValueError("Q1 and Q2 fall on one pixel")
+    q_nx = f((n_q2x - n_q1x) / f(n_qdisp))
+    q_ny = f((n_q2y - n_q1y) / f(n_qdisp))
+    ii = np.arange(n_qdisp)
+    px = np.array([n_q1x + _nint(f(f(k) * q_nx)) for k in ii])
+    py = np.array([n_q1y + _nint(f(f(k) * q_ny)) for k in ii])
+    n_qx = int(plane["n_qx"])
+    if px.min() < 1 or py.min() < 1 or px.max() > n_qx or py.max() > int(plane["n_qy"]):
+        raise ValueError("the cut leaves the map's pixels")
+    return dict(qidx=(px - 1) + n_qx * (py - 1), pixels=np.stack([px, py], 1), n_qdisp=int(n_qdisp),
+                q_range=(float(q_min), float(q_max)), label=label, q_nx=float(q_nx), q_ny=float(q_ny))
+
+
+def cut_step(plane, q1_3d, q2_3d, q_step):
+    """dq_p of an E(Q) stack (mp_sqom56.f90:1300-1305): q_step times the unit
+    in-plane normal of Q2 - Q1, with the reference's sign."""
+    f = np.float32
+    q_v = np.asarray(plane["q_v"], f)
+    dq = ((np.asarray(q2_3d, f) - q_v) - (np.asarray(q1_3d, f) - q_v)).astype(f)
+    ev = np.asarray(plane["ev"], f)
+    p = np.array([dq[1] * ev[2] - dq[2] * ev[1], dq[2] * ev[0] - dq[0] * ev[2], dq[0] * ev[1] - dq[1] * ev[0]], f)
+    p = (f(-1.0) * p / f(np.sqrt(np.dot(p, p)))).astype(f)
+    return (f(q_step) * p).astype(f)
+
+
+def speckle_filter(S, s_trig, cut=False):
+    """The reference's speckle filter (mp_sqom56.f90:1530-1570) on a copy of
+    a host map: an E = const map [n_qy, n_qx] is filtered x fastest, then y
+    fastest; an E(Q) cut [n_f, n_qdisp] (all n_freq_max frequencies, as the
+    reference filters them) once, frequency fastest.  s_trig 0 is no filter."""
+    a = np.array(S, np.float64, order="C", copy=True)
+    if a.ndim != 2:
+        raise ValueError("speckle_filter takes a 2-D map")
+    L = _lib.load()
+    p = a.ctypes.data_as(C.c_void_p)
+    for walk in ((1,) if cut else (0, 1)):
+        _lib.check("speckle_filter", L.mpsq_speckle_filter(p, a.shape[0], a.shape[1], walk, float(s_trig)))
+    return a

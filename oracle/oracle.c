@@ -36,6 +36,14 @@
  *   orc_sqw_direct          T=1 closed form, constant amplitudes -> S==0,
  *                           pure-tone closed form (peak at m0), numpy FFT of
  *                           the windowed series, Parseval.
+ *   orc_iqt                 constant amplitudes -> 0 (mean subtraction); a
+ *                           pure tone a e^{-i w t} -> n_int |a|^2 cos(w lag)
+ *                           (scaled: |a|^2 cos(w lag)); lag 0 -> sum of |C'|^2;
+ *                           equals numpy's correlation of each window.
+ *   orc_speckle_pass        known answers: a lone spike / dip takes its
+ *                           neighbours' max / min, s_trig 1 flattens, and a
+ *                           hand-worked vector whose result depends on the
+ *                           sequential order of the pass.
  *   orc_es_spread           end-to-end NUFFT against orc_dirft_type1 at eps.
  */
 #include <math.h>
@@ -533,6 +541,104 @@ void orc_stage(int dim, int64_t M, const float *pos, const int32_t *sel,
 /*           n_int <= 1 -> window 1                                          */
 /* subtract_mean = 0 skips the M_j term (the pair mode raw sums, :1460-1463) */
 /* ------------------------------------------------------------------------ */
+/* C(t,Q) and C'(t,Q) of one Q (mp_sqom56.f90:1338-1377): the weighted
+ * species sum of F, and the same of F - <F>_t (or of F) times qfac.
+ * cc, cp: interleaved complex [T]. */
+static void orc_combine_q(int64_t T, int64_t NQ, int nsp, const double *F, const double *wsc,
+                          const double *wtab, const int32_t *mask, const double *qfac,
+                          int subtract_mean, int64_t q, double *cc, double *cp)
+{
+    double mean[2 * 64];
+    double *mj = (nsp <= 64) ? mean : (double *)malloc((size_t)(2 * nsp) * sizeof(double));
+    for (int j = 0; j < nsp; ++j) {
+        double ar = 0.0, ai = 0.0;
+        for (int64_t t = 0; t < T; ++t) {
+            ar += F[2 * ((t * nsp + j) * NQ + q)];
+            ai += F[2 * ((t * nsp + j) * NQ + q) + 1];
+        }
+        mj[2 * j] = ar / (double)T;
+        mj[2 * j + 1] = ai / (double)T;
+    }
+    for (int64_t t = 0; t < T; ++t) {
+        double sr = 0.0, si = 0.0, pr = 0.0, pi = 0.0;
+        for (int j = 0; j < nsp; ++j) {
+            if (!mask[j]) continue;
+            const double wv = wtab ? wtab[(int64_t)j * NQ + q] : wsc[j];
+            const double fr = F[2 * ((t * nsp + j) * NQ + q)];
+            const double fi = F[2 * ((t * nsp + j) * NQ + q) + 1];
+            sr += wv * fr;
+            si += wv * fi;
+            if (subtract_mean) {
+                pr += wv * (fr - mj[2 * j]);
+                pi += wv * (fi - mj[2 * j + 1]);
+            } else {
+                pr += wv * fr;
+                pi += wv * fi;
+            }
+        }
+        const double qf = qfac ? qfac[q] : 1.0;
+        cc[2 * t] = sr;
+        cc[2 * t + 1] = si;
+        cp[2 * t] = pr * qf;
+        cp[2 * t + 1] = pi * qf;
+    }
+    if (mj != mean) free(mj);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Speckle filter (mp_sqom56.f90:1539-1548, one pass): csp(i2) for i2 = 1..n */
+/* in order, i1 = i2 - 1 and i3 = i2 + 1 wrapped into 1..n, with the values */
+/* already replaced earlier in the pass.  v is the pass's flat walk order.   */
+/* ------------------------------------------------------------------------ */
+void orc_speckle_pass(int64_t n, double *v, double s_trig)
+{
+    for (int64_t i2 = 1; i2 <= n; ++i2) {
+        int64_t i1 = i2 - 1, i3 = i2 + 1;
+        if (i1 < 1) i1 += n;
+        if (i3 > n) i3 -= n;
+        const double c1 = v[i1 - 1], c3 = v[i3 - 1];
+        const double mn = c1 < c3 ? c1 : c3, mx = c1 > c3 ? c1 : c3;
+        if (v[i2 - 1] < mn / s_trig) v[i2 - 1] = mn;
+        if (v[i2 - 1] > s_trig * mx) v[i2 - 1] = mx;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* The intermediate function I(Q,t) (j_sq != 1): for lags lag = lag_lo + l,  */
+/*   out[l][Q] = Re sum_{i=0}^{n_int-1} C'(i + lag, Q) conj(C'(i, Q))        */
+/* times 1/n_int when scaled (the E = const map form, mp_sqom56.f90:1467-1471*/
+/* -- cs_plot is REAL, so the real part), unscaled otherwise (the E(Q) cut,  */
+/* :1504-1510), over nat^2 (:1574-1575, no time step for I(Q,t)).  C' as in  */
+/* orc_sqw_direct; 0 <= lag <= T - n_int.                                    */
+/* ------------------------------------------------------------------------ */
+int orc_iqt(int64_t T, int64_t NQ, int nsp, const double *F, const double *wsc, const double *wtab,
+            const int32_t *mask, const double *qfac, int64_t n_int, int64_t lag_lo, int64_t n_lag,
+            int subtract_mean, int scaled, double nat, double *out)
+{
+    if (T < 1 || NQ < 0 || nsp < 1 || n_int < 1 || n_int > T || lag_lo < 0 || n_lag < 0 || !(nat > 0) ||
+        lag_lo + n_lag - 1 > T - n_int)
+        return 2;
+#pragma omp parallel
+    {
+        double *cc = (double *)malloc((size_t)(2 * T) * sizeof(double));
+        double *cp = (double *)malloc((size_t)(2 * T) * sizeof(double));
+#pragma omp for schedule(static)
+        for (int64_t q = 0; q < NQ; ++q) {
+            orc_combine_q(T, NQ, nsp, F, wsc, wtab, mask, qfac, subtract_mean, q, cc, cp);
+            for (int64_t l = 0; l < n_lag; ++l) {
+                const int64_t lag = lag_lo + l;
+                double acc = 0.0;
+                for (int64_t i = 0; i < n_int; ++i)
+                    acc += cp[2 * (i + lag)] * cp[2 * i] + cp[2 * (i + lag) + 1] * cp[2 * i + 1];
+                out[l * NQ + q] = (scaled ? acc / (double)n_int : acc) / (nat * nat);
+            }
+        }
+        free(cc);
+        free(cp);
+    }
+    return 0;
+}
+
 int orc_sqw_direct(int64_t T, int64_t NQ, int nsp, const double *F,
                    const double *wsc, const double *wtab, const int32_t *mask,
                    const double *qfac, int64_t n_int, int win_kind, double dt,
@@ -567,42 +673,9 @@ int orc_sqw_direct(int64_t T, int64_t NQ, int nsp, const double *F,
         double *cp = (double *)malloc((size_t)(2 * T) * sizeof(double));
 #pragma omp for schedule(static)
         for (int64_t q = 0; q < NQ; ++q) {
-            double mean[2 * 64];
-            double *mj = (nsp <= 64) ? mean : (double *)malloc((size_t)(2 * nsp) * sizeof(double));
-            for (int j = 0; j < nsp; ++j) {
-                double ar = 0.0, ai = 0.0;
-                for (int64_t t = 0; t < T; ++t) {
-                    ar += F[2 * ((t * nsp + j) * NQ + q)];
-                    ai += F[2 * ((t * nsp + j) * NQ + q) + 1];
-                }
-                mj[2 * j] = ar / (double)T;
-                mj[2 * j + 1] = ai / (double)T;
-            }
+            orc_combine_q(T, NQ, nsp, F, wsc, wtab, mask, qfac, subtract_mean, q, cc, cp);
             double tot = 0.0;
-            for (int64_t t = 0; t < T; ++t) {
-                double sr = 0.0, si = 0.0, pr = 0.0, pi = 0.0;
-                for (int j = 0; j < nsp; ++j) {
-                    if (!mask[j]) continue;
-                    const double wv = wtab ? wtab[(int64_t)j * NQ + q] : wsc[j];
-                    const double fr = F[2 * ((t * nsp + j) * NQ + q)];
-                    const double fi = F[2 * ((t * nsp + j) * NQ + q) + 1];
-                    sr += wv * fr;
-                    si += wv * fi;
-                    if (subtract_mean) {
-                        pr += wv * (fr - mj[2 * j]);
-                        pi += wv * (fi - mj[2 * j + 1]);
-                    } else {
-                        pr += wv * fr;
-                        pi += wv * fi;
-                    }
-                }
-                const double qf = qfac ? qfac[q] : 1.0;
-                cc[2 * t] = sr;
-                cc[2 * t + 1] = si;
-                cp[2 * t] = pr * qf;
-                cp[2 * t + 1] = pi * qf;
-                tot += sr * sr + si * si;
-            }
+            for (int64_t t = 0; t < T; ++t) tot += cc[2 * t] * cc[2 * t] + cc[2 * t + 1] * cc[2 * t + 1];
             if (stot) stot[q] = tot / (double)T * inv_n2;
             for (int64_t mi = 0; mi < n_f; ++mi) {
                 double acc = 0.0;
@@ -618,7 +691,6 @@ int orc_sqw_direct(int64_t T, int64_t NQ, int nsp, const double *F,
                 }
                 s[mi * NQ + q] = acc * norm_w;
             }
-            if (mj != mean) free(mj);
         }
         free(cc);
         free(cp);

@@ -57,6 +57,9 @@ def lib():
         L.orc_stage.restype = None
         L.orc_sqw_direct.argtypes = [i64, i64, i32, vp, vp, vp, vp, vp, i64, i32, dbl, dbl,
                                      i64, i64, i32, vp, vp]
+        L.orc_iqt.argtypes = [i64, i64, i32, vp, vp, vp, vp, vp, i64, i64, i64, i32, i32, dbl, vp]
+        L.orc_speckle_pass.argtypes = [i64, vp, dbl]
+        L.orc_speckle_pass.restype = None
         L.orc_es_spread.argtypes = [i32, i64, vp, vp, vp, vp, vp, i32, dbl, vp]
         L.orc_num_threads.restype = i32
         L.orc_set_num_threads.argtypes = [i32]
@@ -250,6 +253,31 @@ def sqw_direct(F, wsc=None, wtab=None, mask=None, qfac=None, n_int=None, win_kin
 # FINUFFT-method restatement on the CPU (the "host FINUFFT-method
 # restatement" timed beside the GPU; FINUFFT itself is not available).
 # ---------------------------------------------------------------------------
+
+def iqt(F, wsc=None, wtab=None, mask=None, qfac=None, n_int=None, lag_lo=0, n_lag=1, subtract_mean=True,
+        scaled=True, nat=1.0):
+    """FP64 intermediate function I(Q,t): [n_lag, NQ] (see orc_iqt)."""
+    F = _c128(F)
+    T, nsp, NQ = F.shape
+    n_int = T // 2 if n_int is None else int(n_int)
+    wsc_a = _f64(wsc if wsc is not None else np.ones(nsp))
+    wtab_a = None if wtab is None else _f64(wtab).reshape(nsp, NQ)
+    mask_a = np.ascontiguousarray(mask if mask is not None else np.ones(nsp), np.int32)
+    qfac_a = None if qfac is None else _f64(qfac).reshape(NQ)
+    out = np.zeros((n_lag, NQ))
+    rc = lib().orc_iqt(T, NQ, nsp, _p(F), _p(wsc_a), _p(wtab_a), _p(mask_a), _p(qfac_a), int(n_int), int(lag_lo),
+                       int(n_lag), int(bool(subtract_mean)), int(bool(scaled)), float(nat), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_iqt rc={rc}")
+    return out
+
+
+def speckle_pass(v, s_trig):
+    """One sequential pass of the speckle filter over a flat array (copy)."""
+    v = _f64(np.array(v, np.float64, copy=True).ravel())
+    lib().orc_speckle_pass(v.size, _p(v), float(s_trig))
+    return v
+
 
 def es_params(eps: float, single: bool = False):
     """Kernel width and shape for upsampling factor 2, after the published
