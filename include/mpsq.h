@@ -358,6 +358,20 @@ int mpsq_server_destroy(mpsq_server srv);
 int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t n_modes[3], const float dq[3],
                   const float a_cell_inv[9], float q_range, float q_step, int n_hist, double *d_hist, void *stream);
 
+/* mpsq_sql_hist without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, result and error convention as mpsq_sql_hist; stream must
+ * belong to the current device.  The kernels' partial histograms live in a
+ * static device array of 2^20 doubles (8 MiB) that each GPU the library runs
+ * on holds until the process ends; a call uses up to 2 x (compute units)
+ * slices of n_hist * npairs doubles, and n_hist * npairs above 8192 (the
+ * workgroup's LDS histogram) is rejected with MPSQ_ERR_SIZE.  Every call on
+ * one GPU shares that array: calls that overlap on one GPU race. */
+int mpsq_sql_hist_noalloc(const void *d_amp, int nsp, int precision, const int64_t n_modes[3], const float dq[3],
+                          const float a_cell_inv[9], float q_range, float q_step, int n_hist, double *d_hist,
+                          void *stream);
+
 /* Library version string. */
 const char *mpsq_version(void);
 

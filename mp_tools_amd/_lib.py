@@ -41,7 +41,7 @@ EXPORTS = [
     "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
     "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
     "mpsq_server_result", "mpsq_server_copy_result", "mpsq_server_release", "mpsq_server_wait_idle",
-    "mpsq_server_stats", "mpsq_server_destroy", "mpsq_sql_hist",
+    "mpsq_server_stats", "mpsq_server_destroy", "mpsq_sql_hist", "mpsq_sql_hist_noalloc",
 ]
 STAGES = ("stage", "sort", "spread_xfft", "colfft", "final_colfft", "reduce")
 
@@ -175,6 +175,7 @@ def load():
     L.mpsq_server_destroy.argtypes = [vp]
     flt = C.c_float
     L.mpsq_sql_hist.argtypes = [vp, i32, i32, vp, vp, vp, flt, flt, i32, vp, vp]
+    L.mpsq_sql_hist_noalloc.argtypes = [vp, i32, i32, vp, vp, vp, flt, flt, i32, vp, vp]
     _lib = L
     return L
 

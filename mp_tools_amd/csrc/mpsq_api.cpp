@@ -2002,13 +2002,13 @@ void mpsq_default_opts(mpsq_opts *o)
 
 const char *mpsq_version(void) { return "mpsq 0.1 (gfx950)"; }
 
-int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
-                  const float *a_cell_inv, float q_range, float q_step, int n_hist, double *d_hist, void *stream)
+static int sql_geom(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
+                    const float *a_cell_inv, float q_range, float q_step, int n_hist, const double *d_hist, SqlGeom &g)
 {
     if (!d_amp || !n_modes || !dq || !a_cell_inv || !d_hist || nsp < 1 || nsp > 8 || n_hist < 1 ||
         !(q_step > 0.0f) || (precision != MPSQ_FP32 && precision != MPSQ_FP64))
         return MPSQ_ERR_BAD_ARG;
-    SqlGeom g{};
+    g = SqlGeom{};
     int64_t nm = 1;
     for (int d = 0; d < 3; ++d) {
         if (n_modes[d] < 1 || n_modes[d] > (int64_t)1 << 20) return MPSQ_ERR_BAD_ARG;
@@ -2023,6 +2023,28 @@ int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t *n_mo
     g.nsp = nsp;
     g.npairs = nsp * (nsp + 1) / 2;
     g.nmodes = nm;
+    return 0;
+}
+
+int mpsq_sql_hist_noalloc(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
+                          const float *a_cell_inv, float q_range, float q_step, int n_hist, double *d_hist,
+                          void *stream)
+{
+    SqlGeom g;
+    const int bad = sql_geom(d_amp, nsp, precision, n_modes, dq, a_cell_inv, q_range, q_step, n_hist, d_hist, g);
+    if (bad) return bad;
+    const int rc = launch_sql_hist_static(d_amp, precision == MPSQ_FP64, g, d_hist, static_cast<hipStream_t>(stream));
+    if (rc == -1 || rc == -3) return MPSQ_ERR_SIZE;
+    if (rc != 0 || hipGetLastError() != hipSuccess) return MPSQ_ERR_DEVICE;
+    return 0;
+}
+
+int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
+                  const float *a_cell_inv, float q_range, float q_step, int n_hist, double *d_hist, void *stream)
+{
+    SqlGeom g;
+    const int bad = sql_geom(d_amp, nsp, precision, n_modes, dq, a_cell_inv, q_range, q_step, n_hist, d_hist, g);
+    if (bad) return bad;
     const int rc = launch_sql_hist(d_amp, precision == MPSQ_FP64, g, d_hist, static_cast<hipStream_t>(stream));
     if (rc == -1) return MPSQ_ERR_SIZE;
     if (rc != 0 || hipGetLastError() != hipSuccess) return MPSQ_ERR_DEVICE;

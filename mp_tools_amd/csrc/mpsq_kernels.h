@@ -223,6 +223,10 @@ struct SqlGeom {
 // returns 0, -1 if the histogram does not fit one workgroup's LDS (64 KB)
 // or nsp > 8, -2 if the partial buffer could not be allocated
 int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st);
+// the same with its partial slices in a static device array (no allocation);
+// returns 0, -1 (LDS), -2 (no scratch address), -3 (scratch too small)
+int launch_sql_hist_static(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st);
+int64_t sql_scratch_doubles();
 // p[0..n) = 0 by a kernel (graph-capturable, unlike a memset node)
 void launch_zero_i32(int *p, int n, hipStream_t st);
 // amax[t] = max over the M strengths of transform t of max(|Re|, |Im|)

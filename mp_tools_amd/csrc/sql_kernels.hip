@@ -16,6 +16,7 @@
 // near the axes) and writes it to its own slice of a partial buffer; a
 // second kernel adds the slices in a fixed order.
 #include <algorithm>
+#include <mutex>
 
 #include "mpsq_kernels.h"
 
@@ -92,6 +93,65 @@ int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, h
                            static_cast<const float2 *>(amp), g, part);
     hipLaunchKernelGGL(k_sql_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, part, nparts, nb, out);
     (void)hipFreeAsync(part, st);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The same histogram without any allocation: the partial slices live in
+// `scratch`, a file-scope device array of SQL_SCRATCH_DOUBLES doubles
+// (8 MiB) that every GPU the library runs on holds until the process ends.
+// Host code never uses the array's name as a device address: once per
+// device, on the first call with that device current, hipGetSymbolAddress
+// gives that device's copy, kept in scratch_ptr[device].  Every call on one
+// GPU shares the array (calls that overlap on one GPU race); each slice is
+// written whole by its workgroup before the reduction reads it, so nothing
+// needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr int64_t SQL_SCRATCH_DOUBLES = (int64_t)1 << 20;
+static __device__ __align__(256) double scratch[SQL_SCRATCH_DOUBLES];
+
+constexpr int SQL_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[SQL_MAX_DEVICES] = {};
+
+// the device's copy of `scratch` (nullptr if the lookup fails)
+static double *scratch_on(int dev)
+{
+    if (dev < 0 || dev >= SQL_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+        g_scratch_ptr[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
+int64_t sql_scratch_doubles() { return SQL_SCRATCH_DOUBLES; }
+
+// Returns 0; -1 when the histogram does not fit one workgroup's LDS (or nsp
+// > 8); -3 when the scratch array would hold fewer than 64 slices; -2 when
+// the device's scratch address cannot be had.
+int launch_sql_hist_static(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st)
+{
+    const int nb = g.n_hist * g.npairs;
+    const size_t lds = sizeof(double) * (size_t)nb;
+    if (lds > 64 * 1024 || g.nsp > 8) return -1;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return -2;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t fit = SQL_SCRATCH_DOUBLES / std::max(nb, 1);
+    if (fit < 64) return -3;
+    const int nparts = (int)std::min<int64_t>((int64_t)std::max(1, ncu) * 2, fit);
+    double *part = scratch_on(dev);
+    if (!part) return -2;
+    if (fp64)
+        hipLaunchKernelGGL((k_sql_hist<double2>), dim3(nparts), dim3(256), lds, st,
+                           static_cast<const double2 *>(amp), g, part);
+    else
+        hipLaunchKernelGGL((k_sql_hist<float2>), dim3(nparts), dim3(256), lds, st,
+                           static_cast<const float2 *>(amp), g, part);
+    hipLaunchKernelGGL(k_sql_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, part, nparts, nb, out);
     return 0;
 }
 
