@@ -132,7 +132,8 @@ __device__ __forceinline__ void overlap(int l, int r0, int b, int w, int n, int 
 //   FFT tile      [rows][RSX] complex R, row r at r * RSX (a wide pass uses
 //                 the first G rows);
 //   narrow cells  packed 8-byte cells of row r at [r * RSX, r * RSX + RS),
-//                 inside the row's own FFT slot;
+//                 counted from byte narrow_bias(w) (so that a lane's base
+//                 address minus 8w is never negative, see k_spread_narrow);
 //   wide planes   one pass of G rows: [2][G][RS] 8-byte cells;
 // then the n1 twiddles.
 __host__ __device__ inline int spread_row_stride(int n1, int w) { return (n1 + w + 1) & ~1; }
@@ -156,11 +157,12 @@ template <typename R> int spread_group_rows(int rows, int n1)
     return g;
 }
 // bytes of the tile region (accumulators / FFT rows); the twiddles follow it
+__host__ __device__ constexpr int narrow_bias(int w) { return (8 * w + 15) / 16 * 16; }
 template <typename R> size_t spread_tile_bytes(int rows, int n1, int w)
 {
     const size_t G = (size_t)spread_group_rows<R>(rows, n1);
     const size_t RSX = (size_t)spread_row_stride_x(n1, w);
-    const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * 8 : 0;
+    const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * 8 + (size_t)narrow_bias(w) : 0;
     const size_t wideb = std::max(2 * G * spread_row_stride(n1, w) * 8, G * RSX * sizeof(typename cplx<R>::T));
     return (std::max(narrowb, wideb) + 15) / 16 * 16;
 }
@@ -451,7 +453,23 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
     const int xcd = blockIdx.x & 7;
     const int xbeg = xr[xcd], xend = xr[xcd + 1];
     // packed cells: im * 2^32 + re (two's complement, mod 2^64), row r at r * RSX
-    unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem_raw);
+    unsigned long long *acc = reinterpret_cast<unsigned long long *>(smem_raw + narrow_bias(W));
+    // Lane order of a stencil's cells.  Near-lattice inputs put many lanes of
+    // a wave on cells of one LDS bank at once (atoms of a lattice row lie a
+    // fixed number of cells apart, and the rows are bank-aligned), so lane
+    // group g = lane mod NG adds its W cells in the rotated order
+    // lx + (m + rot) mod W, rot = g * (W / NG): lanes that start on the same
+    // cell use NG different banks at every atomic instruction.  The sums are
+    // integers, so the order does not change them.  In the unrolled cell loop
+    // the wrap (m + rot >= W) is a compile-time function of m per group, so a
+    // row needs NG base addresses and every atomic keeps an immediate offset:
+    // cell m of the lane is at base[seg(m)] + 8 m with the lane constant
+    // rofs[s] = 8 rot - (g >= NG - s ? 8 W : 0) (>= -8 W: hence the bias).
+    constexpr int NG = (W >= 4) ? 4 : (W >= 2 ? 2 : 1), RSTEP = W / NG;
+    const int rg = lane & (NG - 1), rot = rg * RSTEP;
+    int rofs[NG];
+#pragma unroll
+    for (int s = 0; s < NG; ++s) rofs[s] = 8 * rot - ((rg >= NG - s) ? 8 * W : 0);
     C *tile = reinterpret_cast<C *>(smem_raw);
 
     if (threadIdx.x == 0) S.next = xbeg + atomicAdd(wc + xcd * XCD_CTR_STRIDE, 1);
@@ -492,7 +510,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         if (ok && am > 0.0f) q = 29 - ilogb((double)max(cb, 1) * (double)am);
 
         if (ok) {
-            uint4 *a4 = reinterpret_cast<uint4 *>(smem_raw);
+            uint4 *a4 = reinterpret_cast<uint4 *>(acc);
             for (int i = threadIdx.x; i < rows * RSX / 2; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
         } else if (threadIdx.x == 0) {
             const int np = rows >> J.grp_sh;  // wide passes of G rows
@@ -530,10 +548,14 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 // kernel values at u = (d + m) * 2/w: x for all W cells (FP64
                 // for the cell fmas), y only for the tile's rows (nyb <= by,
                 // uniform), z one per z row inside the row loop
+                // kxd[m]: the kernel at the lane's m-th cell lx + (m + rot) mod W
                 double kxd[W];
                 const float ux = p.dx * sw;
 #pragma unroll
-                for (int m = 0; m < W; ++m) kxd[m] = (double)es_kernel_u(fmaf((float)m, sw, ux), bl);
+                for (int m = 0; m < W; ++m) {
+                    const int mm = (m + rot < W) ? m + rot : m + rot - W;
+                    kxd[m] = (double)es_kernel_u(fmaf((float)mm, sw, ux), bl);
+                }
                 float ky[W];
                 const float uy = (p.dy + (float)my) * sw;
 #pragma unroll
@@ -554,14 +576,21 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                             const double vr = (double)(csx * wyz), vi = (double)(csy * wyz);
                             unsigned char *ab = reinterpret_cast<unsigned char *>(acc) +
                                                 (__mul24(rr0 + iy, RSX8) + abase);
-                            unsigned long long *ac = reinterpret_cast<unsigned long long *>(ab);
+                            unsigned char *base[NG];
+#pragma unroll
+                            for (int s = 0; s < NG; ++s) base[s] = ab + rofs[s];
 #pragma unroll
                             for (int m = 0; m < W; ++m) {
+                                // segment of m: groups g >= NG - seg have wrapped
+                                int seg = 0;
+#pragma unroll
+                                for (int g = 1; g < NG; ++g) seg += (m >= W - g * RSTEP) ? 1 : 0;
                                 const unsigned long long br =
                                     (unsigned long long)__double_as_longlong(__builtin_fma(vr, kxd[m], FIX_MAGIC));
                                 const unsigned long long bi =
                                     (unsigned long long)__double_as_longlong(__builtin_fma(vi, kxd[m], FIX_MAGIC_IM));
-                                __hip_atomic_fetch_add(ac + m, pack_cell(br, bi), __ATOMIC_RELAXED,
+                                __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(base[seg] + 8 * m),
+                                                       pack_cell(br, bi), __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_WORKGROUP);
                             }
                         }
