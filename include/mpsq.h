@@ -189,6 +189,53 @@ typedef struct mpsq_reduce_spec {
     int64_t n_qidx;
 } mpsq_reduce_spec;
 
+/* The one-phonon approximation of the amplitudes (j_oneph = 1,
+ * mp_sqom56.f90:1037-1098) for lattice (CELL) data: per frame and atom type
+ * j, the displacements d = pos - base_j - at_ind + n_row / 2 summed over the
+ * vertical cell index into the columns u_x, u_y(k, l) of the Q plane ((hk0):
+ * (k, l) = the cell's (a, b) index, d its (x, y); (hhl): k = (a + b - 2) mod
+ * (n_row(1) + n_row(2)) / 2 + 1, l = c, d_x = (d_a + d_b + 2) / sqrt 2, d_y =
+ * d_c + 1), U = 2-D forward FFT(u) / n_row(1), and for the pixels of n_bz x
+ * n_bz zones (x fastest: n_qx = n_bz n_row(1) pixels of 1 / n_row(1)
+ * r.l.u. from qx_min, n_qy likewise)
+ *   A_j(Q) = exp(-i 2 pi Q . base_j) (q_x U_x(kk, ll) + q_y U_y(kk, ll)),
+ * (kk, ll) = the pixel's position inside its zone shifted by the first
+ * pixel's (:1083-1086); (hhl) phases use (base_a + base_b, base_c).  Atoms at
+ * (0, 0, 0) are vacancies.  The reference's REAL(4) steps (displacements, Q,
+ * phase argument, zone offset) are kept; sums, the DFT and the products are
+ * FP64.  n_row(1) = n_row(2) (and = n_row(3) for (hhl)), as the reference's
+ * loops assume; a cell index outside the columns is MPSQ_ERR_BAD_ARG. */
+typedef struct mpsq_oneph_spec {
+    int n_row[3];             /* cells along a, b, c */
+    int n_bz;                 /* Brillouin zones per axis */
+    int j_plane;              /* 1: (hk0), 2: (hhl) */
+    int nsp;                  /* atom types: d_pos holds nsp blocks of n_row(1) n_row(2) n_row(3) atoms */
+    float qx_min, qy_min;     /* first pixel of the map (r.l.u.) */
+    const float *at_base;     /* host [nsp][3] fractional basis positions */
+    int precision;            /* MPSQ_FP32: complex float output, MPSQ_FP64: complex double */
+} mpsq_oneph_spec;
+/* d_pos: device float [T][n_tot][4] (CELL positions in lattice units, as the
+ * engine takes them); d_at_ind: device int32 [n_tot][4] (1-based cell indices
+ * and atom type); d_amp: device [T][nsp][NQ] (wsc and d_wtab NULL) or the
+ * species combined [T][NQ] with host weights wsc[nsp] or device per-Q weights
+ * d_wtab[nsp][NQ] (mp_sqom56.f90:1338-1356).  Synchronises. */
+int mpsq_oneph_amplitudes(const mpsq_oneph_spec *spec, const float *d_pos, const int32_t *d_at_ind, int64_t T,
+                          const double *wsc, const float *d_wtab, void *d_amp, void *stream);
+
+/* Vibrational spectra of mp_dos (mp_dos56.f90:495-527): for every atom and
+ * velocity component the Hann-windowed forward FFT over the T frames
+ * (window 0.5 (1 - cos(2 pi (t + 1) / (T + 1))), t = 0 .. T-1, :495-498;
+ * no mean subtraction), and per atom type s and frequency f < n_freq
+ *   d_dos[f][s] = sum over atoms of type s of sum_c |FFT(v_c w)(f)|^2 / T
+ * (cs_plot of :519-525, type = at_ind .w of the atom, 1-based).  d_vel:
+ * device float [T][n_tot][4] (v_x, v_y, v_z, unused), d_at_ind: device int32
+ * [n_tot][4]; n_freq <= T / 2 + 1 (the reference's n_freq_plot, int(0.5 (T -
+ * 1) t_step / t_step) + 1 in REAL(4)).  MPSQ_FP32 computes the FFT in FP32
+ * (the reference's own precision), MPSQ_FP64 in FP64; the sums are FP64 in a
+ * fixed order.  Synchronises. */
+int mpsq_dos(const float *d_vel, int64_t T, int64_t n_tot, const int32_t *d_at_ind, int nsp, int64_t n_freq,
+             int precision, double *d_dos, void *stream);
+
 /* The reference's per-Q tables of a Q plane (mp_sqom56.f90:909-936),
  * computed on the device in its REAL(4) arithmetic: for pixel (i, j),
  * i = 0..n_qx-1 fastest (the CMCL order of the plane's amplitudes, k1 =

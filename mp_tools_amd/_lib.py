@@ -36,7 +36,7 @@ EXPORTS = [
     "mpsq_sqw_default_spec", "mpsq_sqw_create", "mpsq_sqw_amplitudes", "mpsq_sqw_reduce",
     "mpsq_sqw_destroy", "mpsq_sqw_sync", "mpsq_sqw_nq", "mpsq_sqw_nact", "mpsq_version",
     "mpsq_sqw_set_timing", "mpsq_sqw_get_timing", "mpsq_sqw_debug_sorted", "mpsq_sqw_info", "mpsq_qtables",
-    "mpsq_speckle_filter",
+    "mpsq_speckle_filter", "mpsq_oneph_amplitudes", "mpsq_dos",
     "mpsq_batch_create", "mpsq_batch_run", "mpsq_batch_set_budget", "mpsq_batch_set_timing",
     "mpsq_batch_get_timing", "mpsq_batch_destroy", "mpsq_autotune_tiles",
     "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
@@ -88,6 +88,11 @@ class Request(C.Structure):
     _fields_ = [("traj", C.c_int), ("mask", C.c_void_p), ("n_modes", C.c_int64 * 3), ("f0", C.c_int64),
                 ("T", C.c_int64), ("eps", C.c_double), ("weight_kind", C.c_int), ("w_scalar", C.c_void_p),
                 ("xff", C.c_void_p), ("dt", C.c_double), ("n_f", C.c_int64)]
+
+
+class OnephSpec(C.Structure):
+    _fields_ = [("n_row", C.c_int * 3), ("n_bz", C.c_int), ("j_plane", C.c_int), ("nsp", C.c_int),
+                ("qx_min", C.c_float), ("qy_min", C.c_float), ("at_base", C.c_void_p), ("precision", C.c_int)]
 
 
 class ReduceSpec(C.Structure):
@@ -154,6 +159,8 @@ def load():
     L.mpsq_sqw_debug_sorted.argtypes = [vp, vp, i64, vp, vp, vp, i64]
     L.mpsq_qtables.argtypes = [vp, i32, vp, vp, vp, vp]
     L.mpsq_speckle_filter.argtypes = [vp, i64, i64, i32, C.c_double]
+    L.mpsq_oneph_amplitudes.argtypes = [vp, vp, vp, i64, vp, vp, vp, vp]
+    L.mpsq_dos.argtypes = [vp, i64, i64, vp, i32, i64, i32, vp, vp]
     L.mpsq_sqw_info.argtypes = [vp, vp]
     L.mpsq_sqw_get_timing.argtypes = [vp, vp, vp, i32]
     L.mpsq_batch_create.argtypes = [i32, vp, vp]

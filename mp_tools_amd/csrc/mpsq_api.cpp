@@ -1992,6 +1992,71 @@ struct mpsq_sqw_s {
     }
 };
 
+// mp_dos spectra: the velocity columns' windowed time FFT through the
+// S(Q,w) reduction kernels (one window over all frames, no mean subtraction,
+// Hann window of the (i + 1) / (n + 1) form, norm 1 / T), then the species
+// sums; chunks of columns keep the |FFT|^2 rows within 512 MB.
+template <typename R>
+static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t *d_at_ind, int nsp, int64_t n_freq,
+                    double *d_dos, hipStream_t st)
+{
+    const int64_t ncs = 2 * n_tot;  // complex columns per frame: (v_x, v_y), (v_z, pad)
+    const int64_t n_f = 2 * n_freq - 1;
+    SqwParams p{};
+    p.T = T;
+    p.kind = 0;
+    p.n_int = (int)T;
+    p.n_frame = 1;
+    p.m_lo = -(n_freq - 1);
+    p.n_f = (int)n_f;
+    p.win_kind = 1;
+    p.subtract_mean = 0;
+    p.norm_s = 1.0 / (double)T;
+    p.norm_tot = 0.0;
+    RadixPlan rp{};
+    const bool single = (T <= sqw_tile_elems<R>()) && sqw_single_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
+    DevBuf tw, win, S, buf;
+    int cols = 1;
+    if (single) {
+        std::vector<Cx<R>> twh;
+        upload_twiddles<R>(tw, (int)T, st, twh);
+        std::vector<double> wh((size_t)T);
+        for (int64_t t = 0; t < T; ++t)
+            wh[t] = (T <= 1) ? 1.0 : 0.5 * (1.0 - std::cos(2.0 * M_PI * (double)(t + 1) / (double)(T + 1)));
+        win.ensure(sizeof(double) * (size_t)T);
+        HIPCHK(hipMemcpyAsync(win.p, wh.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS &&
+               sqw_single_lds<R>(T, 2 * cols) <= SQW_MAX_LDS)
+            cols *= 2;
+    }
+    const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(ncs, ((int64_t)512 << 20) / (8 * n_f)));
+    S.ensure(sizeof(double) * (size_t)(n_f * chunk));
+    if (sizeof(R) == 8) buf.ensure(sizeof(double2) * (size_t)(T * chunk));
+    launch_zero_i32(reinterpret_cast<int *>(d_dos), (int)(2 * n_freq * nsp), st);
+    for (int64_t c0 = 0; c0 < ncs; c0 += chunk) {
+        const int64_t nc = std::min(chunk, ncs - c0);
+        p.NQ = nc;
+        const Cx<R> *amp;
+        if (sizeof(R) == 8) {
+            launch_cols_to_f64(d_vel, T, ncs, c0, nc, buf.as<double>(), st);
+            amp = buf.as<Cx<R>>();
+            p.NQs = nc;
+        } else {
+            amp = reinterpret_cast<const Cx<R> *>(d_vel) + c0;
+            p.NQs = ncs;
+        }
+        if (single)
+            launch_sqw_single<R>(p, rp, cols, amp, tw.as<Cx<R>>(), win.as<double>(), nullptr, S.as<double>(), nullptr,
+                                 st);
+        else
+            launch_sqw_sliding<R>(p, amp, nullptr, S.as<double>(), nullptr, st);
+        launch_dos_species(S.as<double>(), nc, c0, n_freq, d_at_ind, nsp, d_dos, st);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+}
+
 extern "C" {
 
 void mpsq_default_opts(mpsq_opts *o)
@@ -2340,6 +2405,107 @@ int mpsq_sqw_info(mpsq_sqw e, int64_t *out)
     const Geo &g = e->ed ? e->ed->core.g : e->ef->core.g;
     const int64_t v[8] = {g.n1, g.n2, g.n3, g.w, g.by, g.bz, g.BY, g.BZ};
     for (int k = 0; k < 8; ++k) out[k] = v[k];
+    return 0;
+}
+
+int mpsq_dos(const float *d_vel, int64_t T, int64_t n_tot, const int32_t *d_at_ind, int nsp, int64_t n_freq,
+             int precision, double *d_dos, void *stream)
+{
+    if (!d_vel || !d_at_ind || !d_dos || T < 1 || n_tot < 1 || nsp < 1 || n_freq < 1 || n_freq > T / 2 + 1 ||
+        (precision != MPSQ_FP32 && precision != MPSQ_FP64) || 2 * n_freq * nsp > ((int64_t)1 << 30))
+        return MPSQ_ERR_BAD_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPSQ_ERR_DEVICE;
+    try {
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        if (precision == MPSQ_FP64) dos_run<double>(d_vel, T, n_tot, d_at_ind, nsp, n_freq, d_dos, st);
+        else dos_run<float>(d_vel, T, n_tot, d_at_ind, nsp, n_freq, d_dos, st);
+    } catch (const Err &er) {
+        return er.code;
+    } catch (...) {
+        return MPSQ_ERR_DEVICE;
+    }
+    return 0;
+}
+
+int mpsq_oneph_amplitudes(const mpsq_oneph_spec *sp, const float *d_pos, const int32_t *d_at_ind, int64_t T,
+                          const double *wsc, const float *d_wtab, void *d_amp, void *stream)
+{
+    if (!sp || !d_pos || !d_at_ind || !d_amp || T < 1 || !sp->at_base) return MPSQ_ERR_BAD_ARG;
+    const int n1 = sp->n_row[0], n2 = sp->n_row[1], n3 = sp->n_row[2];
+    if (n1 < 1 || n2 < 1 || n3 < 1 || n1 > 1024 || n2 > 1024 || n3 > 1024 || sp->n_bz < 1 || sp->nsp < 1 ||
+        sp->nsp > ONEPH_MAX_SP || (sp->j_plane != 1 && sp->j_plane != 2) ||
+        (sp->precision != MPSQ_FP32 && sp->precision != MPSQ_FP64))
+        return MPSQ_ERR_BAD_ARG;
+    // the reference's loops read the (n_row(1), n_row(2)) columns with the
+    // zone pixels of either axis: square for (hk0), cubic for (hhl)
+    if (n1 != n2 || (sp->j_plane == 2 && n3 != n1)) return MPSQ_ERR_BAD_ARG;
+    const int64_t NQ = (int64_t)sp->n_bz * n1 * sp->n_bz * n2;
+    if (NQ > ((int64_t)1 << 31)) return MPSQ_ERR_SIZE;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MPSQ_ERR_DEVICE;
+    try {
+        OnephParams p{};
+        for (int d = 0; d < 3; ++d) p.n_row[d] = sp->n_row[d];
+        p.n_bz = sp->n_bz;
+        p.j_plane = sp->j_plane;
+        p.nsp = sp->nsp;
+        p.combine = (wsc || d_wtab) ? 1 : 0;
+        p.nsuper = (int64_t)n1 * n2 * n3;
+        p.n_tot = p.nsuper * sp->nsp;
+        p.qx_min = sp->qx_min;
+        p.qy_min = sp->qy_min;
+        p.twopi_s = (float)(2.0 * M_PI);
+        p.sqrt2 = std::sqrt(2.0f);
+        // zone offset: n_row * (q_min - floor(q_min)) in REAL(4), truncated (:1039-1040)
+        p.nqx_min = (int)((float)n1 * (sp->qx_min - std::floor(sp->qx_min)));
+        p.nqy_min = (int)((float)n2 * (sp->qy_min - std::floor(sp->qy_min)));
+        for (int j = 0; j < sp->nsp; ++j)
+            for (int c = 0; c < 3; ++c) p.base[j][c] = sp->at_base[3 * j + c];
+        hipStream_t st = static_cast<hipStream_t>(stream);
+        // frames per chunk: the three work grids within ~256 MB
+        const int64_t per_frame = (int64_t)sp->nsp * 2 * n1 * n2 * (8 + 16 + 16);
+        const int nfc = (int)std::max<int64_t>(1, std::min<int64_t>(T, ((int64_t)256 << 20) / per_frame));
+        DevBuf u, ua, U, tw1, tw2, wd, bad;
+        u.ensure(sizeof(double) * (size_t)nfc * sp->nsp * 2 * n1 * n2);
+        ua.ensure(sizeof(double2) * (size_t)nfc * sp->nsp * 2 * n1 * n2);
+        U.ensure(sizeof(double2) * (size_t)nfc * sp->nsp * 2 * n1 * n2);
+        auto table = [&](DevBuf &b, int n) {
+            std::vector<double2> h((size_t)n);
+            for (int m = 0; m < n; ++m) h[m] = make_double2(std::cos(2.0 * M_PI * m / n), -std::sin(2.0 * M_PI * m / n));
+            b.ensure(sizeof(double2) * (size_t)n);
+            HIPCHK(hipMemcpyAsync(b.p, h.data(), sizeof(double2) * (size_t)n, hipMemcpyHostToDevice, st));
+            HIPCHK(hipStreamSynchronize(st));
+        };
+        table(tw1, n1);
+        table(tw2, n2);
+        if (wsc) {
+            wd.ensure(sizeof(double) * (size_t)sp->nsp);
+            HIPCHK(hipMemcpyAsync(wd.p, wsc, sizeof(double) * (size_t)sp->nsp, hipMemcpyHostToDevice, st));
+        }
+        bad.ensure(sizeof(int));
+        launch_zero_i32(bad.as<int>(), 1, st);
+        for (int64_t f0 = 0; f0 < T; f0 += nfc) {
+            const int nf = (int)std::min<int64_t>(nfc, T - f0);
+            if (sp->precision == MPSQ_FP64)
+                launch_oneph<double>(p, d_pos, d_at_ind, f0, nf, u.as<double>(), ua.as<double2>(), U.as<double2>(),
+                                     tw1.as<double2>(), tw2.as<double2>(), wd.as<double>(), d_wtab, d_amp,
+                                     bad.as<int>(), st);
+            else
+                launch_oneph<float>(p, d_pos, d_at_ind, f0, nf, u.as<double>(), ua.as<double2>(), U.as<double2>(),
+                                    tw1.as<double2>(), tw2.as<double2>(), wd.as<double>(), d_wtab, d_amp,
+                                    bad.as<int>(), st);
+            HIPCHK(hipGetLastError());
+        }
+        int hbad = 0;
+        HIPCHK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hbad & 4) return MPSQ_ERR_BAD_ARG;
+    } catch (const Err &er) {
+        return er.code;
+    } catch (...) {
+        return MPSQ_ERR_DEVICE;
+    }
     return 0;
 }
 

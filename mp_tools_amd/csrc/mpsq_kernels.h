@@ -260,9 +260,33 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 template <typename R>
 void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S,
                         double *Stot, hipStream_t st);
+// DOS: dos[f][s] += species sums of (S(f) + S(-f)) / 2 over columns [c0, c0 + ncols)
+void launch_dos_species(const double *S, int64_t ncols, int64_t c0, int64_t n_freq, const int *at_ind, int nsp,
+                        double *dos, hipStream_t st);
+void launch_cols_to_f64(const float *in, int64_t T, int64_t ncs, int64_t c0, int64_t nc, double *out, hipStream_t st);
 // I(Q,t): S[l][q] = norm Re sum_{i<n_int} C'(i + m_lo + l) conj C'(i), and S_tot
 template <typename R>
 void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,
                 hipStream_t st);
+
+// One-phonon amplitudes (oneph_kernels.hip)
+constexpr int ONEPH_MAX_SP = 64;
+struct OnephParams {
+    int n_row[3];
+    int n_bz, j_plane, nsp, combine;
+    int64_t nsuper, n_tot;
+    float qx_min, qy_min;
+    float twopi_s, sqrt2;   // the reference's REAL(4) 2 pi and sqrt(2)
+    int nqx_min, nqy_min;   // zone offset of the first pixel (truncated)
+    float base[ONEPH_MAX_SP][3];
+};
+// frames [f0, f0 + nf): u (nf * nsp * 2 * n1 * n2 doubles), ua and U (as
+// many double2) are work arrays; tw1 / tw2: exp(-2 pi i m / n) for n_row(1),
+// n_row(2); out per species or combined (p.combine: wsc or wtab); bad |= 4
+// when a cell index lies outside the columns
+template <typename R>
+void launch_oneph(const OnephParams &p, const float *pos, const int *at_ind, int64_t f0, int nf, double *u,
+                  double2 *ua, double2 *U, const double2 *tw1, const double2 *tw2, const double *wsc,
+                  const float *wtab, void *out, int *bad, hipStream_t st);
 
 }  // namespace mpsq

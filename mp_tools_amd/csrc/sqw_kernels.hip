@@ -398,6 +398,61 @@ void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float 
     hipLaunchKernelGGL((k_iqt<R, MG>), dim3(gx, gy > 0 ? gy : 1), dim3(256), 0, st, p, amp, qfac, S, Stot);
 }
 
+// Vibrational DOS species sums (mp_dos56.f90:519-525) from |FFT|^2 of
+// packed velocity columns: column q of the chunk is complex column c0 + q of
+// [n_tot][2] per frame, i.e. atom (c0 + q) / 2 with (v_x + i v_y) or
+// (v_z + i 0).  For a pair of real signals packed as z = x + i y,
+// |X(f)|^2 + |Y(f)|^2 = (|Z(f)|^2 + |Z(-f)|^2) / 2, which also holds for
+// (v_z + i 0).  S rows hold m = -(n_freq - 1) .. n_freq - 1.
+//   dos[f][s] += sum over the chunk's columns of atoms of type s (at_ind .w,
+//                1-based) of (S(f) + S(-f)) / 2
+// Block (f, s): threads sum the columns in a fixed stride order, then a
+// fixed tree; the chunks are added in order, so the sums are deterministic.
+__global__ __launch_bounds__(256) void k_dos_species(const double *__restrict__ S, int64_t ncols, int64_t c0,
+                                                     int64_t n_freq, const int4 *__restrict__ at_ind, int nsp,
+                                                     double *__restrict__ dos)
+{
+    __shared__ double red[256];
+    const int64_t f = blockIdx.x;
+    const int s = blockIdx.y;
+    const double *Sp = S + (n_freq - 1 + f) * ncols, *Sm = S + (n_freq - 1 - f) * ncols;
+    double acc = 0.0;
+    for (int64_t q = threadIdx.x; q < ncols; q += 256)
+        if (at_ind[(c0 + q) >> 1].w - 1 == s) acc += 0.5 * (Sp[q] + Sm[q]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dos[f * nsp + s] += red[0];
+}
+
+void launch_dos_species(const double *S, int64_t ncols, int64_t c0, int64_t n_freq, const int *at_ind, int nsp,
+                        double *dos, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dos_species, dim3((unsigned)n_freq, (unsigned)nsp), dim3(256), 0, st, S, ncols, c0, n_freq,
+                       reinterpret_cast<const int4 *>(at_ind), nsp, dos);
+}
+
+// complex128 copy of complex columns [c0, c0 + nc) of float [T][ncs] pairs
+__global__ void k_cols_to_f64(const float2 *__restrict__ in, int64_t T, int64_t ncs, int64_t c0, int64_t nc,
+                              double2 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T * nc) return;
+    const int64_t t = i / nc, q = i - t * nc;
+    const float2 v = in[t * ncs + c0 + q];
+    out[i] = make_double2((double)v.x, (double)v.y);
+}
+
+void launch_cols_to_f64(const float *in, int64_t T, int64_t ncs, int64_t c0, int64_t nc, double *out, hipStream_t st)
+{
+    if (T * nc == 0) return;
+    hipLaunchKernelGGL(k_cols_to_f64, dim3((unsigned)((T * nc + 255) / 256)), dim3(256), 0, st,
+                       reinterpret_cast<const float2 *>(in), T, ncs, c0, nc, reinterpret_cast<double2 *>(out));
+}
+
 template <typename R> struct SqwTiles;
 template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
 template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };

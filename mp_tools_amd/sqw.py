@@ -553,3 +553,85 @@ def speckle_filter(S, s_trig, cut=False):
     for walk in ((1,) if cut else (0, 1)):
         _lib.check("speckle_filter", L.mpsq_speckle_filter(p, a.shape[0], a.shape[1], walk, float(s_trig)))
     return a
+
+
+def oneph_amplitudes(pos, at_ind, at_base, n_row, n_bz, j_plane=1, q_min=(0.0, 0.0), precision="fp32",
+                     weights=None, w_table=None, stream=None):
+    """One-phonon amplitudes (mp_sqom56.f90:1037-1098) of lattice (CELL)
+    data on the GPU: ``pos`` float32 CUDA [T, n_tot, 4] (nsp blocks of
+    n_row(1) n_row(2) n_row(3) atoms), ``at_ind`` int32 CUDA [n_tot, 4]
+    (1-based cells), ``at_base`` [nsp, 3].  Returns [T, nsp, NQ] or, with
+    scalar ``weights`` [nsp] or a per-Q ``w_table`` [nsp, NQ] (CUDA float32),
+    the combined [T, NQ]; NQ = (n_bz n_row(1)) x (n_bz n_row(2)), x fastest."""
+    import torch
+
+    if not (pos.is_cuda and at_ind.is_cuda):
+        raise ValueError("pos and at_ind must be CUDA tensors")
+    base = np.ascontiguousarray(at_base, np.float32).reshape(-1, 3)
+    nsp = base.shape[0]
+    T = pos.shape[0]
+    n_row = [int(v) for v in n_row]
+    NQ = (n_bz * n_row[0]) * (n_bz * n_row[1])
+    s = _lib.OnephSpec()
+    s.n_row[:] = n_row
+    s.n_bz, s.j_plane, s.nsp = int(n_bz), int(j_plane), nsp
+    s.qx_min, s.qy_min = float(q_min[0]), float(q_min[1])
+    s.at_base = base.ctypes.data
+    s.precision = _lib.FP64 if precision == "fp64" else _lib.FP32
+    cdt = torch.complex128 if precision == "fp64" else torch.complex64
+    combine = weights is not None or w_table is not None
+    out = torch.empty((T, NQ) if combine else (T, nsp, NQ), dtype=cdt, device=pos.device)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+    wt = None if w_table is None else w_table.to(torch.float32).contiguous()
+    pos = pos.contiguous()
+    ind = at_ind.to(torch.int32).contiguous()
+    rc = _lib.load().mpsq_oneph_amplitudes(C.byref(s), pos.data_ptr(), ind.data_ptr(), T,
+                                           None if w is None else w.ctypes.data,
+                                           None if wt is None else wt.data_ptr(), out.data_ptr(), stream)
+    _lib.check("oneph_amplitudes", rc)
+    return out
+
+
+def dos_n_freq(nfile, t_step):
+    """mp_dos's frequency count n_freq_plot (mp_dos56.f90:362, 476-477):
+    int(0.5 t_tot / t_step) + 1 with t_tot = (nfile - 1) t_step, REAL(4)."""
+    f = np.float32
+    t_tot = f(nfile - 1) * f(t_step)
+    return int(f(f(0.5) * t_tot) / f(t_step)) + 1
+
+
+def dos_spectra(vel, at_ind, nsp, n_freq, precision="fp32", stream=None):
+    """Per-type vibrational spectra (mp_dos56.f90:495-527) on the GPU:
+    ``vel`` float32 CUDA [T, n_tot, 4] velocities, ``at_ind`` int32 CUDA
+    [n_tot, 4] (type in column 3, 1-based).  Returns float64 CUDA
+    [n_freq, nsp]: sum over the type's atoms and components of
+    |FFT(v w)(f)|^2 / T, w the Hann window 0.5 (1 - cos(2 pi (t+1) / (T+1)))."""
+    import torch
+
+    if not (vel.is_cuda and at_ind.is_cuda):
+        raise ValueError("vel and at_ind must be CUDA tensors")
+    vel = vel.to(torch.float32).contiguous()
+    ind = at_ind.to(torch.int32).contiguous()
+    T, n_tot = vel.shape[0], vel.shape[1]
+    out = torch.empty((int(n_freq), int(nsp)), dtype=torch.float64, device=vel.device)
+    rc = _lib.load().mpsq_dos(vel.data_ptr(), T, n_tot, ind.data_ptr(), int(nsp), int(n_freq),
+                              _lib.FP64 if precision == "fp64" else _lib.FP32, out.data_ptr(), stream)
+    _lib.check("dos", rc)
+    return out
+
+
+def dos_output(cs_plot, weights, occup, nsuper, nfile, t_step, mask=None):
+    """mp_dos's plotted spectra (mp_dos56.f90:571-589): per-type columns
+    cs_plot[f, s] * w_s and their total, times nfile / at_weight_sum with
+    at_weight_sum = sum(occup w nsuper) (sum of the Hann window)^2
+    (t_step / 0.01).  weights: 1 (uniform) or b_coh^2 per type; mask 0/1."""
+    cs = np.asarray(cs_plot, np.float64)
+    nsp = cs.shape[1]
+    w = np.asarray(weights, np.float64) * (np.ones(nsp) if mask is None else np.asarray(mask, np.float64))
+    i = np.arange(1, int(nfile) + 1, dtype=np.float64)
+    wind_sum = np.sum(0.5 * (1.0 - np.cos(2.0 * np.pi * i / (nfile + 1))))
+    aws = np.sum(np.asarray(occup, np.float64) * w * np.asarray(nsuper, np.float64)) * wind_sum ** 2 * (t_step / 0.01)
+    out = np.empty((cs.shape[0], nsp + 1))
+    out[:, :nsp] = cs * w[None, :]
+    out[:, nsp] = out[:, :nsp].sum(axis=1)
+    return out * nfile / aws

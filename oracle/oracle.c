@@ -40,6 +40,13 @@
  *                           pure tone a e^{-i w t} -> n_int |a|^2 cos(w lag)
  *                           (scaled: |a|^2 cos(w lag)); lag 0 -> sum of |C'|^2;
  *                           equals numpy's correlation of each window.
+ *   orc_oneph               numpy's fft2 for the column DFT, a single
+ *                           displaced atom (closed form), a plane-wave
+ *                           displacement pattern (peaks at its wave
+ *                           vector), vacancies and the (hhl) column map.
+ *   orc_dos                 numpy's FFT of the windowed series, a constant
+ *                           velocity (f = 0 closed form), Parseval over the
+ *                           half spectrum of real signals.
  *   orc_speckle_pass        known answers: a lone spike / dip takes its
  *                           neighbours' max / min, s_trig 1 flattens, and a
  *                           hand-worked vector whose result depends on the
@@ -583,6 +590,142 @@ static void orc_combine_q(int64_t T, int64_t NQ, int nsp, const double *F, const
         cp[2 * t + 1] = pi * qf;
     }
     if (mj != mean) free(mj);
+}
+
+/* ------------------------------------------------------------------------ */
+/* One-phonon amplitudes (mp_sqom56.f90:1037-1098), per frame and atom type, */
+/* in the reference's loop order with 1-based indices: displacement columns  */
+/* u_x, u_y(k, l) (REAL(4) displacements, FP64 sums), their 2-D forward DFT  */
+/* (direct double sum) / n_row(1), then every pixel of the n_bz x n_bz       */
+/* zones.  pos float [T][n_tot][4], at_ind int32 [n_tot][4], base float      */
+/* [nsp][3]; out complex128 [T][nsp][NQ], NQ = n_bz n1 x n_bz n2, x fastest.  */
+/* Returns 2 for a cell index outside the columns.                           */
+/* ------------------------------------------------------------------------ */
+int orc_oneph(int64_t T, int nsp, const int32_t *n_row, int n_bz, int j_plane, float qx_min, float qy_min,
+              const float *pos, const int32_t *at_ind, const float *base, double *out)
+{
+    const int n1 = n_row[0], n2 = n_row[1], n3 = n_row[2];
+    const int64_t nsuper = (int64_t)n1 * n2 * n3, n_tot = nsuper * nsp;
+    const int nqx = n_bz * n1, nqy = n_bz * n2;
+    const int64_t NQ = (int64_t)nqx * nqy;
+    const float twopi_s = (float)(2.0 * M_PI), sqrt2 = sqrtf(2.0f);
+    const int nqx_min = (int)((float)n1 * (qx_min - floorf(qx_min)));
+    const int nqy_min = (int)((float)n2 * (qy_min - floorf(qy_min)));
+    double *ux = (double *)malloc(sizeof(double) * (size_t)n1 * n2);
+    double *uy = (double *)malloc(sizeof(double) * (size_t)n1 * n2);
+    double *Ux = (double *)malloc(sizeof(double) * 2 * (size_t)n1 * n2);
+    double *Uy = (double *)malloc(sizeof(double) * 2 * (size_t)n1 * n2);
+    int rc = 0;
+    for (int64_t f = 0; f < T && !rc; ++f)
+        for (int j = 0; j < nsp && !rc; ++j) {
+            memset(ux, 0, sizeof(double) * (size_t)n1 * n2);
+            memset(uy, 0, sizeof(double) * (size_t)n1 * n2);
+            for (int64_t i = 0; i < nsuper; ++i) {
+                const int64_t a = (int64_t)j * nsuper + i;
+                const float *r = pos + 4 * (f * n_tot + a);
+                const int32_t *ind = at_ind + 4 * a;
+                if (!(r[0] != 0.0f || r[1] != 0.0f || r[2] != 0.0f)) continue;
+                int k, l;
+                float dx, dy;
+                if (j_plane == 1) {
+                    k = ind[0];
+                    l = ind[1];
+                    dx = r[0] - base[3 * j] - (float)ind[0] + (float)(n1 / 2);
+                    dy = r[1] - base[3 * j + 1] - (float)ind[1] + (float)(n2 / 2);
+                } else {
+                    const int m = (n1 + n2) / 2;
+                    k = ((ind[0] - 1 + ind[1] - 1) % m + m) % m + 1;
+                    l = ind[2];
+                    dx = r[0] - base[3 * j] - (float)ind[0] + (float)(n1 / 2) + 1.0f;
+                    dx = dx + (r[1] - base[3 * j + 1] - (float)ind[1] + (float)(n2 / 2) + 1.0f);
+                    dx = dx / sqrt2;
+                    dy = r[2] - base[3 * j + 2] - (float)ind[2] + (float)(n3 / 2) + 1.0f;
+                }
+                if (k < 1 || k > n1 || l < 1 || l > n2) {
+                    rc = 2;
+                    break;
+                }
+                ux[(k - 1) + (int64_t)n1 * (l - 1)] += (double)dx;
+                uy[(k - 1) + (int64_t)n1 * (l - 1)] += (double)dy;
+            }
+            if (rc) break;
+            /* U(kk, ll) = sum_{k,l} u(k, l) exp(-2 pi i ((k-1)(kk-1)/n1 + (l-1)(ll-1)/n2)) / n1 */
+            for (int kk = 0; kk < n1; ++kk)
+                for (int ll = 0; ll < n2; ++ll) {
+                    double xr = 0, xi = 0, yr = 0, yi = 0;
+                    for (int k = 0; k < n1; ++k)
+                        for (int l = 0; l < n2; ++l) {
+                            const double ph = -ORC_TWO_PI * ((double)((int64_t)k * kk % n1) / n1 +
+                                                            (double)((int64_t)l * ll % n2) / n2);
+                            const double c = cos(ph), s = sin(ph);
+                            xr += ux[k + (int64_t)n1 * l] * c;
+                            xi += ux[k + (int64_t)n1 * l] * s;
+                            yr += uy[k + (int64_t)n1 * l] * c;
+                            yi += uy[k + (int64_t)n1 * l] * s;
+                        }
+                    Ux[2 * (kk + (int64_t)n1 * ll)] = xr / n1;
+                    Ux[2 * (kk + (int64_t)n1 * ll) + 1] = xi / n1;
+                    Uy[2 * (kk + (int64_t)n1 * ll)] = yr / n1;
+                    Uy[2 * (kk + (int64_t)n1 * ll) + 1] = yi / n1;
+                }
+            for (int ii = 1; ii <= n_bz; ++ii)
+                for (int jj = 1; jj <= n_bz; ++jj)
+                    for (int l = 1; l <= n2; ++l)
+                        for (int k = 1; k <= n1; ++k) {
+                            const float q_x = qx_min + (float)(ii - 1) + (float)(k - 1) / (float)n1;
+                            const float q_y = qy_min + (float)(jj - 1) + (float)(l - 1) / (float)n2;
+                            int kk = (nqx_min + k) % n1, ll = (nqy_min + l) % n2;
+                            if (kk == 0) kk = n1;
+                            if (ll == 0) ll = n2;
+                            float arg;
+                            if (j_plane == 1) arg = q_x * base[3 * j] + q_y * base[3 * j + 1];
+                            else arg = q_x * (base[3 * j] + base[3 * j + 1]) + q_y * base[3 * j + 2];
+                            arg = twopi_s * arg;
+                            const double c = cos((double)arg), s = -sin((double)arg);
+                            const int64_t u0 = 2 * ((kk - 1) + (int64_t)n1 * (ll - 1));
+                            const double ar = q_x * Ux[u0] + q_y * Uy[u0], ai = q_x * Ux[u0 + 1] + q_y * Uy[u0 + 1];
+                            const int64_t X = (int64_t)(ii - 1) * n1 + (k - 1), Y = (int64_t)(jj - 1) * n2 + (l - 1);
+                            double *o = out + 2 * ((f * nsp + j) * NQ + X + (int64_t)nqx * Y);
+                            o[0] = c * ar - s * ai;
+                            o[1] = c * ai + s * ar;
+                        }
+        }
+    free(ux);
+    free(uy);
+    free(Ux);
+    free(Uy);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* mp_dos spectra (mp_dos56.f90:495-525): per atom and component, the direct */
+/* DFT of v(t) w(t) with w = 0.5 (1 - cos(2 pi i / (T + 1))), i = 1 .. T,    */
+/* and out[f][s] = sum over atoms of type s (at_ind[4a + 3], 1-based) of     */
+/* sum_c |X(f)|^2 / T for f < n_freq.  vel float [T][n_tot][4].              */
+/* ------------------------------------------------------------------------ */
+int orc_dos(int64_t T, int64_t n_tot, const float *vel, const int32_t *at_ind, int nsp, int64_t n_freq, double *out)
+{
+    if (T < 1 || n_freq < 1 || n_freq > T) return 2;
+    memset(out, 0, sizeof(double) * (size_t)(n_freq * nsp));
+    double *w = (double *)malloc(sizeof(double) * (size_t)T);
+    for (int64_t t = 0; t < T; ++t) w[t] = 0.5 * (1.0 - cos(ORC_TWO_PI * (double)(t + 1) / (double)(T + 1)));
+    for (int64_t a = 0; a < n_tot; ++a) {
+        const int s = at_ind[4 * a + 3] - 1;
+        if (s < 0 || s >= nsp) continue;
+        for (int c = 0; c < 3; ++c)
+            for (int64_t f = 0; f < n_freq; ++f) {
+                double xr = 0.0, xi = 0.0;
+                for (int64_t t = 0; t < T; ++t) {
+                    const double v = (double)vel[4 * (t * n_tot + a) + c] * w[t];
+                    const double ph = -ORC_TWO_PI * (double)((f * t) % T) / (double)T;
+                    xr += v * cos(ph);
+                    xi += v * sin(ph);
+                }
+                out[f * nsp + s] += (xr * xr + xi * xi) / (double)T;
+            }
+    }
+    free(w);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

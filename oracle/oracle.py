@@ -59,6 +59,8 @@ def lib():
                                      i64, i64, i32, vp, vp]
         L.orc_iqt.argtypes = [i64, i64, i32, vp, vp, vp, vp, vp, i64, i64, i64, i32, i32, dbl, vp]
         L.orc_speckle_pass.argtypes = [i64, vp, dbl]
+        L.orc_dos.argtypes = [i64, i64, vp, vp, i32, i64, vp]
+        L.orc_oneph.argtypes = [i64, i32, vp, i32, i32, C.c_float, C.c_float, vp, vp, vp, vp]
         L.orc_speckle_pass.restype = None
         L.orc_es_spread.argtypes = [i32, i64, vp, vp, vp, vp, vp, i32, dbl, vp]
         L.orc_num_threads.restype = i32
@@ -277,6 +279,35 @@ def speckle_pass(v, s_trig):
     v = _f64(np.array(v, np.float64, copy=True).ravel())
     lib().orc_speckle_pass(v.size, _p(v), float(s_trig))
     return v
+
+
+def oneph(pos, at_ind, base, n_row, n_bz, j_plane=1, q_min=(0.0, 0.0)):
+    """FP64 one-phonon amplitudes [T, nsp, NQ] (see orc_oneph)."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    T = pos.shape[0]
+    base = np.ascontiguousarray(base, np.float32).reshape(-1, 3)
+    nsp = base.shape[0]
+    nr = np.ascontiguousarray(n_row, np.int32)
+    ind = np.ascontiguousarray(at_ind, np.int32)
+    NQ = n_bz * int(nr[0]) * n_bz * int(nr[1])
+    out = np.zeros((T, nsp, NQ), np.complex128)
+    rc = lib().orc_oneph(T, nsp, _p(nr), int(n_bz), int(j_plane), float(q_min[0]), float(q_min[1]),
+                         _p(pos), _p(ind), _p(base), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_oneph rc={rc}")
+    return out
+
+
+def dos(vel, at_ind, nsp, n_freq):
+    """FP64 mp_dos spectra [n_freq, nsp] (see orc_dos)."""
+    vel = np.ascontiguousarray(vel, np.float32)
+    T, n_tot = vel.shape[0], vel.shape[1]
+    ind = np.ascontiguousarray(at_ind, np.int32)
+    out = np.zeros((int(n_freq), int(nsp)))
+    rc = lib().orc_dos(T, n_tot, _p(vel), _p(ind), int(nsp), int(n_freq), _p(out))
+    if rc:
+        raise RuntimeError(f"orc_dos rc={rc}")
+    return out
 
 
 def es_params(eps: float, single: bool = False):
