@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-frames-1core", type=int, default=2)
     ap.add_argument("--no-c3", action="store_true", help="skip the 128^3 precision-sweep record")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving record")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the record of the cut / I(Q,t) / one-phonon / DOS paths")
     ap.add_argument("--serve-rates", type=str, default="10,50,150")
     ap.add_argument("--serve-requests", type=int, default=40, help="requests per rate of the serving record")
     ap.add_argument("--seed", type=int, default=20261204)
@@ -265,6 +267,8 @@ def main():
         S_sel["fp64"] = (S64[:, qidx].cpu().numpy(), St64[qidx].cpu().numpy())
         del out64, eng64
         torch.cuda.empty_cache()
+    if not a.no_next:
+        res["next_rows"] = next_rows_record(a, dev, eng32, out32["amp"], pos, counts, nat, dt)
     del eng32
     amp_head = out32["amp"][:max(1, a.cpu_frames)].cpu().numpy()
     del out32
@@ -290,6 +294,80 @@ def main():
         if srv is not None:
             res["serving"]["served_map_check"] = served_map_check(srv)
     print(json.dumps(res))
+
+
+def _timed(fn, reps=3):
+    """Median wall time (ms) of fn() on the current stream, synchronised."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append((time.perf_counter() - t) * 1e3)
+    return float(np.median(ts))
+
+
+def next_rows_record(a, dev, eng, amp, pos, counts, nat, dt):
+    """Times of the paths either side of the S(Q,w) map on the same workload:
+    an E(Q) cut and I(Q,t) (map at 4 lags, cut at every lag) from the C2
+    amplitudes, the one-phonon amplitudes of the C2 lattice (5 sites x
+    40^3 cells, 64 frames) and the mp_dos spectra of its velocities (central
+    differences of the positions, all frames).  Algorithmic bytes: amplitudes / positions /
+    velocities read once plus the outputs written."""
+    import torch
+    from mp_tools_amd import synth
+    from mp_tools_amd import sqw as S
+    T, NQ = amp.shape
+    rec = {}
+    g = S.qplane((40, 40, 40), 2, (1, 0, 0), (0, 1, 0))
+    # a Q line across the plane's pixels, mapped onto the first 256 x 256 modes (kz = 0 plane)
+    N = (256, 256, 32)
+    cut = S.qcut(dict(g, n_qx=N[0], n_qy=N[1]), (-0.9, -0.6, 0.0), (0.95, 0.7, 0.0), (128, 128, 128))
+    qi = torch.tensor(cut["qidx"] + N[0] * N[1] * (N[2] // 2), dtype=torch.int32, device=dev)
+    nq = int(qi.numel())
+    n_int, n_fr = S.reference_n_int(T, 0, sqw=True)
+    ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, nat=nat, dt=dt, qidx=qi))
+    rec["eq_cut"] = {"pixels": nq, "n_int": n_int, "frequencies": n_fr, "ms": ms,
+                     "alg_GBps": (T * nq * 8 + n_fr * nq * 8) / (ms * 1e-3) / 1e9}
+    n_i, n_lag = S.reference_n_int(T, 0, sqw=False)
+    ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, kind=2, nat=nat, qidx=qi))
+    rec["iqt_cut"] = {"pixels": nq, "n_int": n_i, "lags": n_lag, "ms": ms}
+    ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, kind=1, m_lo=0, n_f=4, nat=nat), reps=2)
+    rec["iqt_map_4_lags"] = {"modes": NQ, "n_int": n_i, "ms": ms,
+                             "alg_GBps": (T * NQ * 8 + 5 * NQ * 8) / (ms * 1e-3) / 1e9}
+    # one-phonon amplitudes of the lattice (CELL layout: 5 site blocks)
+    R, _ = synth.perovskite_lattice(40)
+    cells = np.floor(R + 20.0 + 1e-9).astype(np.int32)
+    at_ind = np.zeros((R.shape[0], 4), np.int32)
+    at_ind[:, :3] = cells + 1
+    at_ind[:, 3] = np.repeat(np.arange(1, 6), 40 ** 3)
+    dind = torch.tensor(at_ind, device=dev)
+    nfr = min(T, 64)
+    ms = _timed(lambda: S.oneph_amplitudes(pos[:nfr], dind, synth.SITES, (40, 40, 40), 2, 1, (0.0, 0.0),
+                                           weights=np.array([5.07, 7.16, 5.80, 5.80, 5.80]) * 0.1))
+    npx = (2 * 40) ** 2
+    rec["one_phonon"] = {"frames": nfr, "atoms": int(R.shape[0]), "pixels": npx, "ms": ms,
+                         "ms_per_frame": ms / nfr,
+                         "alg_GBps": nfr * (R.shape[0] * 16 + npx * 8) / (ms * 1e-3) / 1e9}
+    # mp_dos spectra of the velocities (central differences, one-sided at the ends)
+    vel = torch.zeros_like(pos)
+    vel[1:-1, :, :3] = (pos[2:, :, :3] - pos[:-2, :, :3]) / (2 * dt)
+    vel[0, :, :3] = (pos[1, :, :3] - pos[0, :, :3]) / dt
+    vel[-1, :, :3] = (pos[-1, :, :3] - pos[-2, :, :3]) / dt
+    ntype = 3
+    ind3 = torch.tensor(np.repeat(np.arange(1, 4), counts).astype(np.int32)[:, None] * np.array([[0, 0, 0, 1]],
+                                                                                                np.int32),
+                        device=dev)
+    nf = S.dos_n_freq(T, dt)
+    ms = _timed(lambda: S.dos_spectra(vel, ind3, ntype, nf), reps=2)
+    rec["dos"] = {"frames": T, "atoms": int(pos.shape[1]), "frequencies": nf, "ms": ms,
+                  "alg_GBps": T * pos.shape[1] * 16 / (ms * 1e-3) / 1e9}
+    del vel
+    torch.cuda.empty_cache()
+    return rec
 
 
 def direct_dft_check(a, pos, counts, w, qsets, S_sel, geo, n_f, dt, nat):
