@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-frames-1core", type=int, default=2)
     ap.add_argument("--no-c3", action="store_true", help="skip the 128^3 precision-sweep record")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving record")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default spread tile instead of timing the shapes first")
     ap.add_argument("--no-next", action="store_true",
                     help="skip the record of the cut / I(Q,t) / one-phonon / DOS paths")
     ap.add_argument("--serve-rates", type=str, default="10,50,150")
@@ -155,6 +157,15 @@ def main():
         tm = eng.timing()
         return eng, out, el / steps, tm
 
+    # spread tile shapes timed on the first frames and cached per geometry
+    # (mpsq_autotune_tiles), as a user would before a long run; not timed
+    tuned = {}
+    if not a.no_autotune:
+        from mp_tools_amd.sqw import autotune_tiles
+        for prec in ("fp32",) + (() if a.no_fp64 else ("fp64",)):
+            t0 = time.time()
+            by, bz, msf = autotune_tiles(pos, 3, N, counts, eps=a.eps, precision=prec, weights=w, tpe=geo["tpe"])
+            tuned[prec] = {"tile": [by, bz], "ms_per_frame": msf, "s": time.time() - t0}
     eng32, out32, sec32, tm32 = run("fp32", a.steps, a.warmup)
     frames_s = T / sec32
     info32 = eng32.info()
@@ -179,6 +190,7 @@ def main():
                    "fine_grid": [info32["n1"], info32["n2"], info32["n3"]], "kernel_width": info32["w"],
                    "spread_tile": [info32["by"], info32["bz"]]},
         "nu_pts_per_s": T * M / sec32,
+        "autotune": tuned,
         "trajectory_gen_s": gen_s,
     }
     # stage times (HIP events on the engine's stream over the timed steps)

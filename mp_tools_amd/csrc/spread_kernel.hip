@@ -275,6 +275,97 @@ __device__ __forceinline__ void x_fft(const RadixPlan &rp1, int n1, int cnt, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same x FFT with each wave owning whole rows (rows wv, wv + nwv, ...):
+// a row's stages read and write only its own LDS slot, so the stages of a
+// wave are separated by a wave-level wait instead of a workgroup barrier and
+// the waves run their rows independently.  Usable when every wave's rows fit
+// its lanes' registers (ceil(cnt / nwv) * n <= E * 64).
+// ---------------------------------------------------------------------------
+// LDS writes of this wave complete before its next LDS reads (a wave's LDS
+// operations are ordered; the "memory" clobber keeps the compiler from moving
+// loads or stores across)
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
+template <typename C, int E, int R, class Load, class Store>
+__device__ __forceinline__ void stage_io_w(int n, int cntw, int wv, int nwv, FDiv dnb, FDiv dns, int Ns, int tws,
+                                           const C *__restrict__ tw, Load load, Store store, bool sync_mid)
+{
+    constexpr int TPT = (E + R - 1) / R;
+    const int lane = threadIdx.x & 63;
+    const int nb = n / R;
+    const int total = nb * cntw;
+    C v[TPT * R];
+    int tt[TPT], bb[TPT];
+#pragma unroll
+    for (int i = 0; i < TPT; ++i) {
+        const int task = lane + i * 64;
+        bb[i] = -1;
+        if (task < total) {
+            const int tl = (int)fdiv((uint32_t)task, dnb);
+            const int j = task - tl * nb;
+            const int k = j - (int)fdiv((uint32_t)j, dns) * Ns;
+            const int tk = k * tws;
+            const int t = wv + tl * nwv;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                C a = load(t, j + r * nb);
+                if (r > 0 && Ns > 1) a = cmul(a, tw[r * tk]);
+                v[i * R + r] = a;
+            }
+            Bfly<R, C>::run(&v[i * R]);
+            tt[i] = t;
+            bb[i] = (j - k) * R + k;
+        }
+    }
+    if (sync_mid) wave_lds_sync();
+#pragma unroll
+    for (int i = 0; i < TPT; ++i)
+        if (bb[i] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) store(tt[i], bb[i] + r * Ns, v[i * R + r]);
+        }
+}
+
+template <typename C, int E, class Load, class Store>
+__device__ __forceinline__ void stage_any_w(int R, int n, int cntw, int wv, int nwv, FDiv dnb, FDiv dns, int Ns,
+                                            int tws, const C *__restrict__ tw, Load load, Store store, bool sync_mid)
+{
+    switch (R) {
+    case 8: stage_io_w<C, E, 8>(n, cntw, wv, nwv, dnb, dns, Ns, tws, tw, load, store, sync_mid); break;
+    case 4: stage_io_w<C, E, 4>(n, cntw, wv, nwv, dnb, dns, Ns, tws, tw, load, store, sync_mid); break;
+    case 2: stage_io_w<C, E, 2>(n, cntw, wv, nwv, dnb, dns, Ns, tws, tw, load, store, sync_mid); break;
+    case 3: stage_io_w<C, E, 3>(n, cntw, wv, nwv, dnb, dns, Ns, tws, tw, load, store, sync_mid); break;
+    case 5: stage_io_w<C, E, 5>(n, cntw, wv, nwv, dnb, dns, Ns, tws, tw, load, store, sync_mid); break;
+    default: break;
+    }
+}
+
+// rows wv, wv + nwv, ... of cnt; FDiv of n / r per stage as in the plan (the
+// task index counts this wave's rows only)
+template <typename C, int E, class LoadAcc, class LoadTile, class StoreTile, class StoreOut>
+__device__ __forceinline__ void x_fft_w(const RadixPlan &rp1, int n1, int cnt, int wv, int nwv,
+                                        const C *__restrict__ tw, LoadAcc la, LoadTile lt, StoreTile st, StoreOut so)
+{
+    const int cntw = (cnt - wv + nwv - 1) / nwv;
+    if (cntw <= 0) return;
+    int Ns = 1;
+    for (int s = 0; s < rp1.nst; ++s) {
+        const int Rr = rp1.r[s];
+        const bool first = (s == 0), last = (s == rp1.nst - 1);
+        if (first && last)
+            stage_any_w<C, E>(Rr, n1, cntw, wv, nwv, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, la, so, false);
+        else if (first)
+            stage_any_w<C, E>(Rr, n1, cntw, wv, nwv, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, la, st, true);
+        else if (last)
+            stage_any_w<C, E>(Rr, n1, cntw, wv, nwv, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, lt, so, false);
+        else
+            stage_any_w<C, E>(Rr, n1, cntw, wv, nwv, rp1.dnb[s], rp1.dns[s], Ns, rp1.tws[s], tw, lt, st, true);
+        if (!last) wave_lds_sync();
+        Ns *= Rr;
+    }
+}
+
 // binary search of the job owning tile id (tile0s ascending, tile0s[0] = 0)
 __device__ __forceinline__ int job_of(const int *__restrict__ tile0s, int njobs, int id)
 {
@@ -512,6 +603,18 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
         if (ok) {
             uint4 *a4 = reinterpret_cast<uint4 *>(acc);
             for (int i = threadIdx.x; i < rows * RSX / 2; i += NT) a4[i] = make_uint4(0u, 0u, 0u, 0u);
+            // per tile: the kept frequencies' factors 2^-q / phi_hat and the
+            // G1 offset of every tile row, read by the FFT's last stage (the
+            // barrier after the spread makes them visible; the previous
+            // tile's FFT finished reading them at the loop's last barrier)
+            const int N1 = g.N1;
+            float *phs = reinterpret_cast<float *>(smem_raw + spread_tab_offset(J.lds_tw, n1));
+            int64_t *rowoff = reinterpret_cast<int64_t *>(phs + n1 / 2);
+            for (int i = threadIdx.x; i < N1; i += NT) phs[i] = ldexpf(gload(J.phinv1 + i), -q);
+            for (int r = threadIdx.x; r < rows; r += NT) {
+                const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
+                rowoff[r] = (((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1;
+            }
         } else if (threadIdx.x == 0) {
             const int np = rows >> J.grp_sh;  // wide passes of G rows
             const int k = atomicAdd(wc + CTR_FB_COUNT, np);
@@ -612,21 +715,11 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
             c1.fetch(jobs[jn], S, pb ^ 1, vw + NT + lane, tn, total_n, e);
         }
 
-        // ---- x FFT of all rows at once (2E elements per thread): the first
-        // stage reads every group's accumulators before any row is written
+        // ---- x FFT of the tile's rows (2E elements per thread)
         if (ok) {
-            // per tile: the kept frequencies' factors 2^-q / phi_hat and the
-            // G1 offset of every tile row (read by the last stage, after at
-            // least one barrier of the FFT)
             const int N1 = g.N1, h1 = N1 / 2;
-            float *phs = reinterpret_cast<float *>(smem_raw + spread_tab_offset(J.lds_tw, n1));
-            int64_t *rowoff = reinterpret_cast<int64_t *>(phs + n1 / 2);
-            for (int i = threadIdx.x; i < N1; i += NT) phs[i] = ldexpf(gload(J.phinv1 + i), -q);
-            for (int r = threadIdx.x; r < rows; r += NT) {
-                const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
-                rowoff[r] = (((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1;
-            }
-            if (J.rp1.nst == 1) __syncthreads();
+            const float *phs = reinterpret_cast<const float *>(smem_raw + spread_tab_offset(J.lds_tw, n1));
+            const int64_t *rowoff = reinterpret_cast<const int64_t *>(phs + n1 / 2);
             // the ghost cells x = n1 + i fold onto i < w here (32-bit adds
             // wrap; the final sums are exact); 2^-q goes into the output factor
             auto load_acc = [&](int row, int i) -> C {
@@ -649,7 +742,21 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 const float sc = phs[kk];
                 gstore(G1 + rowoff[row] + kk, cmk<C>(v.x * sc, cj ? -(v.y * sc) : v.y * sc));
             };
-            x_fft<C, 2 * E, NT>(J.rp1, n1, rows, tw, load_acc, load_tile, store_tile, store_out);
+            // each wave transforms whole rows when its share fits its lanes'
+            // registers (rows * n1 <= 2E * NT always holds, so a multiple of
+            // 8 rows always does) and when every row's accumulators lie in
+            // the row's own FFT slot (the accumulators start narrow_bias(W)
+            // bytes into the tile, so a row needs that much slack past its
+            // n1 + W cells: else the first stage of one row's wave could
+            // overwrite the next row's last cells before its wave read
+            // them); else all rows at once with workgroup barriers between
+            // the stages, whose first stage reads every row before writing
+            constexpr int NW = NT / 64;
+            if (((rows + NW - 1) / NW) * n1 <= 2 * E * 64 &&
+                narrow_bias(W) + spread_row_stride(n1, W) * 8 <= RSX * 8)
+                x_fft_w<C, 2 * E>(J.rp1, n1, rows, wv, NW, tw, load_acc, load_tile, store_tile, store_out);
+            else
+                x_fft<C, 2 * E, NT>(J.rp1, n1, rows, tw, load_acc, load_tile, store_tile, store_out);
         }
         if (!has_next) break;
         __syncthreads();  // every read of the tile (and the twiddles) done
