@@ -76,6 +76,13 @@ class MPSnapshot:
     at_ind: np.ndarray = None     # int32 [n_tot, 4]
     at_pos: np.ndarray = None     # float32 [n_tot, 4]
     time_stamp: str = ""
+    # further float32 [n_tot, 4] blocks, present as the header says (N_TRAJ >= 1:
+    # velocities, N_TRAJ == 2: forces; J_SHELL_OUT == 1: the shells' blocks)
+    at_vel: np.ndarray = None
+    at_force: np.ndarray = None
+    at_pos_s: np.ndarray = None
+    at_vel_s: np.ndarray = None
+    at_force_s: np.ndarray = None
     groups: list = field(default_factory=list)  # [(name, {key: value})] in record order
 
     @property
@@ -109,8 +116,29 @@ class MPSnapshot:
         return np.diag(1.0 / self.a_par)
 
 
+# the float32 blocks after AT_POS, in file order (mp_lbin56.f90:1490-1547):
+# core velocities (N_TRAJ >= 1), core forces (N_TRAJ == 2), then with
+# J_SHELL_OUT == 1 the shell positions, velocities and forces likewise
+def _blocks(hdr: dict) -> list:
+    n_traj = int(hdr.get("N_TRAJ", 0))
+    shells = int(hdr.get("J_SHELL_OUT", 0)) == 1
+    b = []
+    if n_traj >= 1:
+        b.append("at_vel")
+    if n_traj == 2:
+        b.append("at_force")
+    if shells:
+        b.append("at_pos_s")
+        if n_traj >= 1:
+            b.append("at_vel_s")
+        if n_traj == 2:
+            b.append("at_force_s")
+    return b
+
+
 def read_dat(path: str) -> MPSnapshot:
-    """Read one MP_TOOLS snapshot (header, species indices, positions)."""
+    """Read one MP_TOOLS snapshot: header, species indices, positions and
+    the velocity / force / shell blocks the header announces."""
     raw = open(path, "rb").read()
     rb = 4 * L_REC
     if len(raw) % rb:
@@ -129,7 +157,8 @@ def read_dat(path: str) -> MPSnapshot:
     n_tot, n_rec = int(hdr["N_TOT"]), int(hdr["N_REC"])
     if n_rec * L_REC < 4 * n_tot or (n_rec - 1) * L_REC >= 4 * n_tot:
         raise ValueError(f"{path}: N_REC={n_rec} does not match N_TOT={n_tot} at {L_REC} words per record")
-    if len(raw) < (n_head + 2 * n_rec) * rb:
+    extra = _blocks(hdr)
+    if len(raw) < (n_head + (2 + len(extra)) * n_rec) * rb:
         raise ValueError(f"{path}: truncated ({len(raw)} bytes)")
     o = n_head * rb
     ind = np.frombuffer(raw, np.int32, 4 * n_tot, o).reshape(n_tot, 4).copy()
@@ -137,6 +166,9 @@ def read_dat(path: str) -> MPSnapshot:
     pos = np.frombuffer(raw, np.float32, 4 * n_tot, o).reshape(n_tot, 4).copy()
     s = MPSnapshot(version=version, n_head=n_head, header=hdr, at_ind=ind, at_pos=pos,
                    time_stamp=" ".join(f[3:]), groups=groups)
+    for name in extra:
+        o += n_rec * rb
+        setattr(s, name, np.frombuffer(raw, np.float32, 4 * n_tot, o).reshape(n_tot, 4).copy())
     if int(s.species_counts.sum()) != n_tot:
         raise ValueError(f"{path}: NSUPER_R sums to {s.species_counts.sum()}, N_TOT is {n_tot}")
     return s
@@ -212,9 +244,15 @@ def _array_records(a: np.ndarray) -> bytes:
 
 def dat_bytes(snap: MPSnapshot) -> bytes:
     """The bytes of an MP_TOOLS snapshot file holding snap's header groups,
-    AT_IND and AT_POS (no velocity / force / shell records)."""
-    if int(snap.header.get("N_TRAJ", 0)) != 0 or int(snap.header.get("J_SHELL_OUT", 0)) == 1:
-        raise ValueError("velocities, forces and shells are not written")
+    AT_IND, AT_POS and the blocks N_TRAJ / J_SHELL_OUT announce, in the
+    writer's order (mp_lbin56.f90:1476-1547)."""
+    hdr = {}
+    for _, d in snap.groups:  # the header as written (the groups it holds)
+        hdr.update(d)
+    extra = _blocks(hdr)
+    for name in extra:
+        if getattr(snap, name) is None:
+            raise ValueError(f"the header announces {name}, which the snapshot lacks")
     n_head = 1 + len(snap.groups)
     head = ("MP_TOOLS".ljust(16) + snap.version.ljust(4)[:4] + (" " + "%11d" % n_head).ljust(40)
             + "   " + snap.time_stamp.strip()).ljust(L_REC)
@@ -223,6 +261,8 @@ def dat_bytes(snap: MPSnapshot) -> bytes:
         out.append(_rec(namelist_record(g, d).ljust(L_REC).encode("latin1")))
     out.append(_array_records(np.asarray(snap.at_ind, np.int32)))
     out.append(_array_records(np.asarray(snap.at_pos, np.float32)))
+    for name in extra:
+        out.append(_array_records(np.asarray(getattr(snap, name), np.float32)))
     return b"".join(out)
 
 
@@ -231,12 +271,13 @@ def write_dat(path: str, snap: MPSnapshot) -> None:
         f.write(dat_bytes(snap))
 
 
-def load_trajectory(paths, device="cuda", pin: bool = True):
+def load_trajectory(paths, device="cuda", pin: bool = True, block: str = "at_pos"):
     """Read MP_TOOLS snapshots (one per frame) into one device tensor
     float32 [T, n_tot, 4] (x, y, z, charge; species-major atom order), the
-    layout the S(Q,w) engine takes.  Frames are staged through one pinned
-    host buffer and copied with one transfer.  All files must hold the same
-    atoms (N_TOT and NSUPER_R)."""
+    layout the S(Q,w) engine takes; block "at_vel" (or another of the
+    snapshot's blocks) gives the velocities mpsq_dos takes.  Frames are
+    staged through one pinned host buffer and copied with one transfer.  All
+    files must hold the same atoms (N_TOT and NSUPER_R)."""
     import torch
 
     paths = list(paths)
@@ -247,11 +288,18 @@ def load_trajectory(paths, device="cuda", pin: bool = True):
     host = torch.empty((len(paths), n_tot, 4), dtype=torch.float32,
                        pin_memory=bool(pin and torch.cuda.is_available()))
     hv = host.numpy()
-    hv[0] = first.at_pos
+
+    def take(s, p):
+        a = getattr(s, block)
+        if a is None:
+            raise ValueError(f"{p}: no {block} block (N_TRAJ {s.header.get('N_TRAJ')})")
+        return a
+
+    hv[0] = take(first, paths[0])
     for k, p in enumerate(paths[1:], start=1):
         s = read_dat(p)
         if s.n_tot != n_tot or not np.array_equal(s.species_counts, counts):
             raise ValueError(f"{p}: atoms differ from {paths[0]}")
-        hv[k] = s.at_pos
+        hv[k] = take(s, p)
     return host.to(device, non_blocking=False), first
 

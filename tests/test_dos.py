@@ -86,3 +86,37 @@ def test_dos_gpu_vs_oracle(gpu_lib, precision, tol, T):
     torch.cuda.synchronize()
     assert got.shape == (n_freq, nsp)
     assert rel_l2(got.cpu().numpy(), ref) < tol
+
+
+@pytest.mark.gpu
+def test_dos_from_dat_velocities(gpu_lib, tmp_path):
+    """Snapshots with velocity records (N_TRAJ = 1) written like mp_lbin,
+    loaded to the device by their velocity block, through mpsq_dos, against
+    the oracle on the same arrays."""
+    import os
+    import torch
+    from conftest import rel_l2
+    from mp_tools_amd import mpdat
+    here = os.path.dirname(os.path.abspath(__file__))
+    s = mpdat.read_dat(os.path.join(here, "golden", "al_u", "Al_U_large_n0001.dat"))
+    n, T = 300, 24
+    d1 = dict(s.groups[0][1])
+    d1.update(N_TOT=n, N_REC=-(-4 * n // mpdat.L_REC), N_TRAJ=1, J_SHELL_OUT=0)
+    d2 = dict(s.groups[1][1])
+    d2.update(NSUPER_R=[250, 50])
+    v, _ = _vel(T, n, 2, 4)
+    ind = s.at_ind[:n].copy()
+    ind[:, 3] = np.repeat([1, 2], [250, 50])
+    paths = []
+    for t in range(T):
+        snap = mpdat.MPSnapshot(version=s.version, n_head=s.n_head, time_stamp="2026/01/01 00:00:00",
+                                groups=[("DATA_HEADER_1", d1), ("DATA_HEADER_2", d2), s.groups[2]],
+                                at_ind=ind, at_pos=np.zeros((n, 4), np.float32), at_vel=v[t])
+        p = str(tmp_path / f"v{t}.dat")
+        mpdat.write_dat(p, snap)
+        paths.append(p)
+    vel, first = mpdat.load_trajectory(paths, device="cuda", block="at_vel")
+    nf = sqw.dos_n_freq(T, 0.01)
+    got = sqw.dos_spectra(vel, torch.tensor(first.at_ind, device="cuda"), 2, nf)
+    torch.cuda.synchronize()
+    assert rel_l2(got.cpu().numpy(), O.dos(v, ind, 2, nf)) < 2e-6

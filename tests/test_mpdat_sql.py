@@ -172,3 +172,54 @@ def test_dat_round_trip_and_trajectory_loading(tmp_path):
     mpdat.write_dat(other, s)
     with pytest.raises(ValueError):
         mpdat.load_trajectory([paths[0][0], other], device="cpu")
+
+
+@pytest.mark.parametrize("n_traj,shells", [(1, 0), (2, 0), (1, 1), (2, 1), (0, 1)])
+def test_dat_velocity_force_shell_blocks(tmp_path, n_traj, shells):
+    """The blocks after AT_POS in the writer's order (mp_lbin56.f90:1490-1547)
+    round-trip, and sit where mp_dos reads the velocities: the cores' right
+    after AT_POS, the shells' (n_traj + 1) blocks after it
+    (mp_dos56.f90:429-452)."""
+    s = mpdat.read_dat(os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat"))
+    n = 700
+    rng = np.random.default_rng(n_traj * 10 + shells)
+    d1 = dict(s.groups[0][1])
+    d1.update(N_TOT=n, N_REC=-(-4 * n // mpdat.L_REC), N_TRAJ=n_traj, J_SHELL_OUT=shells)
+    d2 = dict(s.groups[1][1])
+    d2.update(NSUPER_R=[600, 100])
+    blk = lambda: rng.random((n, 4), dtype=np.float32)
+    s2 = mpdat.MPSnapshot(version=s.version, n_head=s.n_head, time_stamp="2026/01/01 00:00:00",
+                          groups=[("DATA_HEADER_1", d1), ("DATA_HEADER_2", d2), s.groups[2]],
+                          at_ind=s.at_ind[:n].copy(), at_pos=blk())
+    names = (["at_vel"] if n_traj >= 1 else []) + (["at_force"] if n_traj == 2 else [])
+    if shells:
+        names += ["at_pos_s"] + (["at_vel_s"] if n_traj >= 1 else []) + (["at_force_s"] if n_traj == 2 else [])
+    for nm in names:
+        setattr(s2, nm, blk())
+    p = str(tmp_path / "b.dat")
+    mpdat.write_dat(p, s2)
+    n_rec = d1["N_REC"]
+    assert os.path.getsize(p) == (4 + (2 + len(names)) * n_rec) * 4 * mpdat.L_REC
+    back = mpdat.read_dat(p)
+    for nm in ["at_pos", "at_vel", "at_force", "at_pos_s", "at_vel_s", "at_force_s"]:
+        want = getattr(s2, nm)
+        got = getattr(back, nm)
+        assert (want is None and got is None) or np.array_equal(got, want), nm
+    raw = open(p, "rb").read()
+    rb = 4 * mpdat.L_REC
+    if n_traj >= 1:
+        # mp_dos: cores' velocities right after AT_POS; shells' after (n_traj + 1) more blocks
+        o = (4 + 2 * n_rec) * rb
+        assert np.array_equal(np.frombuffer(raw, np.float32, 4 * n, o).reshape(n, 4), back.at_vel)
+        if shells:
+            o = (4 + 2 * n_rec + (n_traj + 1) * n_rec) * rb
+            assert np.array_equal(np.frombuffer(raw, np.float32, 4 * n, o).reshape(n, 4), back.at_vel_s)
+        traj, _ = mpdat.load_trajectory([p, p], device="cpu", block="at_vel")
+        assert np.array_equal(traj[1].numpy(), s2.at_vel)
+    else:
+        with pytest.raises(ValueError):
+            mpdat.load_trajectory([p], device="cpu", block="at_vel")
+    s2.at_pos_s = None if shells else s2.at_pos_s
+    if shells:
+        with pytest.raises(ValueError):
+            mpdat.dat_bytes(s2)
