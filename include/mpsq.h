@@ -405,6 +405,26 @@ int mpsq_server_destroy(mpsq_server srv);
 int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t n_modes[3], const float dq[3],
                   const float a_cell_inv[9], float q_range, float q_step, int n_hist, double *d_hist, void *stream);
 
+/* Replaces the Monte Carlo integration of mp_sql (j_acc = 2,
+ * code/src/mp_sql56.f90:1146-1215): n_h chains of n_mc samples (the reference's
+ * n_mc is 10^6, :40).  Sample (j, k) draws a point uniformly in [-1, 1)^3 until
+ * it lies in the unit ball (0 < |p| <= 1), takes a radius fraction r in [0, 1)
+ * and the mode d = nint(h * (p / |p|) * r) with h_d = N_d - (N_d / 2 + 1), all in
+ * FP32; d = 0 adds 1 to d_skip[0], |Q(d)| > q_range adds 1 to d_skip[1], else
+ * Re(A_i A_j*) at d is added to d_hist[nint(|Q| / q_step)][p] as in
+ * mpsq_sql_hist.  The random numbers come from Philox4x32-10 with key
+ * (seed & 0xffffffff, seed >> 32) and counter (j, k, attempt, tag), words 0..2
+ * for the point and word 3 for r, each as (word >> 8) * 2^-24; a sample is a
+ * pure function of (seed, tag, j, k), whatever the launch shape.  (The reference
+ * draws from Fortran's random_number and retries without bound; a sample whose
+ * first 64 attempts miss the ball counts as d = 0 here.)  d_hist and d_skip
+ * (device int64[2]) are accumulated, not zeroed.  MPSQ_ERR_BAD_ARG for n_h < 1,
+ * n_mc < 1, n_h > 2^32 - 1, n_mc > 2^31 - 1 or a null d_skip; otherwise the
+ * arguments and limits of mpsq_sql_hist. */
+int mpsq_sql_hist_mc(const void *d_amp, int nsp, int precision, const int64_t n_modes[3], const float dq[3],
+                     const float a_cell_inv[9], float q_range, float q_step, int n_hist, int64_t n_h, int64_t n_mc,
+                     uint64_t seed, uint32_t tag, double *d_hist, int64_t *d_skip, void *stream);
+
 /* mpsq_sql_hist without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

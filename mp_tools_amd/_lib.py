@@ -42,6 +42,7 @@ EXPORTS = [
     "mpsq_server_create", "mpsq_server_add_trajectory", "mpsq_server_submit", "mpsq_server_poll",
     "mpsq_server_result", "mpsq_server_copy_result", "mpsq_server_release", "mpsq_server_wait_idle",
     "mpsq_server_stats", "mpsq_server_destroy", "mpsq_sql_hist", "mpsq_sql_hist_noalloc",
+    "mpsq_sql_hist_mc",
 ]
 STAGES = ("stage", "sort", "spread_xfft", "colfft", "final_colfft", "reduce")
 
@@ -183,6 +184,8 @@ def load():
     flt = C.c_float
     L.mpsq_sql_hist.argtypes = [vp, i32, i32, vp, vp, vp, flt, flt, i32, vp, vp]
     L.mpsq_sql_hist_noalloc.argtypes = [vp, i32, i32, vp, vp, vp, flt, flt, i32, vp, vp]
+    L.mpsq_sql_hist_mc.argtypes = [vp, i32, i32, vp, vp, vp, flt, flt, i32, i64, i64, C.c_uint64, C.c_uint32,
+                                   vp, vp, vp]
     _lib = L
     return L
 

@@ -2116,6 +2116,25 @@ int mpsq_sql_hist(const void *d_amp, int nsp, int precision, const int64_t *n_mo
     return 0;
 }
 
+int mpsq_sql_hist_mc(const void *d_amp, int nsp, int precision, const int64_t *n_modes, const float *dq,
+                     const float *a_cell_inv, float q_range, float q_step, int n_hist, int64_t n_h, int64_t n_mc,
+                     uint64_t seed, uint32_t tag, double *d_hist, int64_t *d_skip, void *stream)
+{
+    SqlGeom g;
+    const int bad = sql_geom(d_amp, nsp, precision, n_modes, dq, a_cell_inv, q_range, q_step, n_hist, d_hist, g);
+    if (bad) return bad;
+    // chain and sample indices are 32-bit counter words; j + (stride mod n_mc)
+    // must not wrap
+    if (!d_skip || n_h < 1 || n_mc < 1 || n_h > (int64_t)UINT32_MAX || n_mc > (int64_t)INT32_MAX)
+        return MPSQ_ERR_BAD_ARG;
+    const SqlMc mc{seed, tag, n_h, n_mc};
+    const int rc = launch_sql_hist_mc(d_amp, precision == MPSQ_FP64, g, mc, d_hist,
+                                      reinterpret_cast<unsigned long long *>(d_skip), static_cast<hipStream_t>(stream));
+    if (rc == -1) return MPSQ_ERR_SIZE;
+    if (rc != 0 || hipGetLastError() != hipSuccess) return MPSQ_ERR_DEVICE;
+    return 0;
+}
+
 // ---- FP64 plan ----
 int mpsq_makeplan(int type, int dim, const int64_t *n_modes, int iflag, int ntrans, double eps, mpsq_plan *plan,
                   const mpsq_opts *opts)

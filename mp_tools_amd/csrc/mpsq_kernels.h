@@ -223,6 +223,18 @@ struct SqlGeom {
 // returns 0, -1 if the histogram does not fit one workgroup's LDS (64 KB)
 // or nsp > 8, -2 if the partial buffer could not be allocated
 int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st);
+// Monte Carlo integration: n_h chains of n_mc samples drawn by Philox4x32-10
+// under key seed and counter (j, k, attempt, tag)
+struct SqlMc {
+    uint64_t seed;
+    uint32_t tag;
+    int64_t n_h, n_mc;
+};
+// out[n_hist][npairs] += the sampled histogram, skip[0..1] += the counts of
+// samples at d = 0 and beyond q_range; returns 0, -1 (LDS or nsp), -2
+// (partial buffer)
+int launch_sql_hist_mc(const void *amp, bool fp64, const SqlGeom &g, const SqlMc &mc, double *out,
+                       unsigned long long *skip, hipStream_t st);
 // the same with its partial slices in a static device array (no allocation);
 // returns 0, -1 (LDS), -2 (no scratch address), -3 (scratch too small)
 int launch_sql_hist_static(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st);

@@ -8,7 +8,9 @@
      reference stacks 2D transforms over k3, which is the same sum);
   3. the Gauss integration (:1224-1254): Re(A_i A_j*) of every mode with
      |Q| <= q_range into bin nint(|Q| / q_step) -- ``mpsq_sql_hist`` --
-     and its normalisation (:1266-1273);
+     and its normalisation (:1266-1273); or the Monte Carlo integration
+     (:1146-1215, j_acc = 2) over n_h x 10^6 radially sampled modes --
+     ``mpsq_sql_hist_mc`` -- and its normalisation (:1260-1265);
   4. the output functions (:1366-1500): Gaussian smoothing, X-ray /
      neutron / unit weights, F(Q), S(Q), Z(Q) (Faber-Ziman), I(Q),
      unscaled I(Q), as the columns the reference writes (:1812-1829).
@@ -64,12 +66,53 @@ def geometry(a_par, a_cell_inv, n_row, n_tot, q_range: float, q_step: float = 0.
     return SqlGeometry(dq, ainv, tuple(n_qq), q_range, q_step, n_hist, at_volume)
 
 
+N_MC = 1_000_000     # samples per Monte Carlo chain (mp_sql56.f90:40)
+
+
+def n_h_max(geo: SqlGeometry) -> int:
+    """Most Monte Carlo chains per frame, int(.5 prod(n_qq) / n_mc) in REAL(4)
+    (mp_sql56.f90:1063)."""
+    prod = np.float32(1.0)
+    for n in geo.n_qq:
+        prod = np.float32(prod * np.float32(n))
+    return int(np.float32(np.float32(0.5) * prod) / np.float32(N_MC))
+
+
+def mc_normalise(h: np.ndarray, n_sup_total: int, geo: SqlGeometry, nfile: int, n_int: int, skips) -> np.ndarray:
+    """sq_hist from the summed Monte Carlo histogram h [nsp, nsp, n_hist]:
+    amplitudes / sqrt(sum n_sup) (mp_sql56.f90:1153), then bins 2.. over
+    q_step nfile and times q_range / (n_int - n_skip1 - n_skip2)
+    (:1260-1265).  The reference resets the skip counts per frame
+    (:1150-1151) and normalises after the frame loop, so ``skips`` are the
+    last frame's."""
+    sq = np.array(h, np.float64) / float(n_sup_total)
+    sq[:, :, 1:] /= float(geo.q_step) * nfile
+    sq[:, :, 1:] *= float(geo.q_range) / float(n_int - skips[0] - skips[1])
+    return sq
+
+
 def histogram(frames, counts, geo: SqlGeometry, eps: float = 1e-6, precision: str = "fp64",
-              device: int = 0, timing: dict | None = None) -> np.ndarray:
-    """Normalised partial histograms sq_hist[i, j, bin] (Gauss integration,
-    mp_sql56.f90:1224-1273) of the snapshots ``frames`` (each float [n_tot, >=3]
-    positions, species-major with ``counts`` atoms per species)."""
+              device: int = 0, timing: dict | None = None, integration: str = "gauss", n_h: int = 1,
+              seed: int = 1, vary_points: bool = False, n_mc: int = N_MC) -> np.ndarray:
+    """Normalised partial histograms sq_hist[i, j, bin] of the snapshots
+    ``frames`` (each float [n_tot, >=3] positions, species-major with
+    ``counts`` atoms per species): Gauss integration (mp_sql56.f90:1224-1273)
+    or, with integration="mc", the Monte Carlo integration over n_h chains of
+    n_mc samples per frame (:1146-1215, :1260-1265).  The reference's default
+    seeding (j_rand = 1, :1160-1166) restarts every chain from the same seed
+    in each frame, so every frame samples the same modes; vary_points=True
+    draws new ones per frame (the frame index is the generator's tag).
+    ``timing`` receives the device time of the loop ("ms") and, for "mc",
+    the last frame's skip counts ("skips")."""
     import torch
+
+    if integration not in ("gauss", "mc"):
+        raise ValueError("integration must be 'gauss' or 'mc'")
+    mc = integration == "mc"
+    if mc and not 1 <= n_h:
+        raise ValueError("n_h must be >= 1")
+    if mc and n_mc == N_MC and n_h > n_h_max(geo):
+        raise ValueError(f"n_h exceeds the {n_h_max(geo)} chains of 10^6 samples this Q box allows (mp_sql56.f90:1069)")
 
     from .nufft import Plan
     L = _lib.load()
@@ -87,7 +130,10 @@ def histogram(frames, counts, geo: SqlGeometry, eps: float = 1e-6, precision: st
     plans = [Plan(1, geo.n_qq, eps=eps, isign=-1, dtype=dtype, device=device, stream="torch") for _ in range(nsp)]
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(stream)
-    for pos in frames:
+    skip = torch.zeros(2, dtype=torch.int64, device=dev)
+    nfile = 0
+    for ifile, pos in enumerate(frames):
+        nfile += 1
         pos = np.asarray(pos)
         for s in range(nsp):
             r = pos[off[s]:off[s + 1], :3].astype(np.float32)
@@ -101,15 +147,27 @@ def histogram(frames, counts, geo: SqlGeometry, eps: float = 1e-6, precision: st
         nm = (C.c_int64 * 3)(N1, N2, N3)
         dq = (C.c_float * 3)(*[float(v) for v in geo.dq])
         ai = (C.c_float * 9)(*[float(v) for v in geo.a_cell_inv.reshape(-1)])
-        rc = L.mpsq_sql_hist(C.c_void_p(amp.data_ptr()), nsp, 1 if precision == "fp64" else 0, nm, dq, ai,
-                             float(geo.q_range), float(geo.q_step), geo.n_hist, C.c_void_p(hist.data_ptr()),
-                             C.c_void_p(stream.cuda_stream))
-        _lib.check("sql_hist", rc)
+        prec = 1 if precision == "fp64" else 0
+        if mc:
+            skip.zero_()
+            rc = L.mpsq_sql_hist_mc(C.c_void_p(amp.data_ptr()), nsp, prec, nm, dq, ai, float(geo.q_range),
+                                    float(geo.q_step), geo.n_hist, int(n_h), int(n_mc), int(seed),
+                                    ifile if vary_points else 0, C.c_void_p(hist.data_ptr()),
+                                    C.c_void_p(skip.data_ptr()), C.c_void_p(stream.cuda_stream))
+            _lib.check("sql_hist_mc", rc)
+        else:
+            rc = L.mpsq_sql_hist(C.c_void_p(amp.data_ptr()), nsp, prec, nm, dq, ai, float(geo.q_range),
+                                 float(geo.q_step), geo.n_hist, C.c_void_p(hist.data_ptr()),
+                                 C.c_void_p(stream.cuda_stream))
+            _lib.check("sql_hist", rc)
     ev1.record(stream)
     torch.cuda.synchronize(dev)
+    skips = tuple(int(v) for v in skip.cpu().tolist())
     if timing is not None:
         timing["ms"] = ev0.elapsed_time(ev1)
-        timing["frames"] = len(frames)
+        timing["frames"] = nfile
+        if mc:
+            timing["skips"] = skips
     h = hist.cpu().numpy().reshape(geo.n_hist, npairs)
     sq = np.zeros((nsp, nsp, geo.n_hist))
     p = 0
@@ -117,11 +175,13 @@ def histogram(frames, counts, geo: SqlGeometry, eps: float = 1e-6, precision: st
         for j in range(i + 1):
             sq[i, j] = sq[j, i] = h[:, p]
             p += 1
+    if mc:
+        return mc_normalise(sq, int(np.sum(counts)), geo, nfile, int(n_h) * int(n_mc), skips)
     # amplitudes / sum(n_sup) (:1231), then the Gauss normalisation (:1266-1272)
     sq /= float(np.sum(counts)) ** 2
     qh = np.arange(geo.n_hist, dtype=np.float64) * float(geo.q_step)
     sq[:, :, 1:] /= float(geo.q_step) * qh[1:] ** 2
-    sq[:, :, 1:] *= 2.0 * math.pi ** 2 / (len(frames) * geo.at_volume)
+    sq[:, :, 1:] *= 2.0 * math.pi ** 2 / (nfile * geo.at_volume)
     return sq
 
 

@@ -47,6 +47,18 @@
  *   orc_dos                 numpy's FFT of the windowed series, a constant
  *                           velocity (f = 0 closed form), Parseval over the
  *                           half spectrum of real signals.
+ *   orc_philox4x32_10       the published known-answer vectors of the
+ *                           Random123 suite (zero, all-ones and pi-digit
+ *                           counter/key).
+ *   orc_sql_hist_mc         constant amplitudes: every kept sample adds
+ *                           exactly Re(c_i c_j*), so bins hold integer
+ *                           counts that with the skips add up to n_h n_mc;
+ *                           the radius law (counts flat in |Q|); isotropy
+ *                           (direction moments <cos^2> = 1/3, <cos^4> = 1/5
+ *                           of the unit sphere, which cube corners would
+ *                           break); the normalised estimate of a smooth
+ *                           |A(Q)|^2 against its value and against the
+ *                           Gauss sum over the same grid.
  *   orc_speckle_pass        known answers: a lone spike / dip takes its
  *                           neighbours' max / min, s_trig 1 flattens, and a
  *                           hand-worked vector whose result depends on the
@@ -725,6 +737,134 @@ int orc_dos(int64_t T, int64_t n_tot, const float *vel, const int32_t *at_ind, i
             }
     }
     free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as  */
+/* easy as 1, 2, 3", SC'11): ten rounds of                                   */
+/*   (c0, c1, c2, c3) <- (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2),                    */
+/*                        hi(M0 c0) ^ c3 ^ k1, lo(M0 c0))                    */
+/* with M0 = 0xD2511F53, M1 = 0xCD9E8D57 and the key bumped by the Weyl      */
+/* constants (0x9E3779B9, 0xBB67AE85) after each round.                      */
+/* ------------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static float orc_unit(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
+
+/* |Q| of the mode offset (k1, k2, k3) in FP32 (mp_sql56.f90:1036-1044):      */
+/* norm2(a_cell_inv . (k * d_q)), every operation rounded on its own.        */
+static float orc_sql_qnorm(const float *dq, const float *ainv, int k1, int k2, int k3)
+{
+    const float u[3] = {(float)k1 * dq[0], (float)k2 * dq[1], (float)k3 * dq[2]};
+    float v[3];
+    for (int a = 0; a < 3; ++a) v[a] = (ainv[3 * a] * u[0] + ainv[3 * a + 1] * u[1]) + ainv[3 * a + 2] * u[2];
+    return sqrtf((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* mp_sql's Monte Carlo integration (j_acc = 2, mp_sql56.f90:1177-1210) for  */
+/* one frame: chains k < n_h of n_mc samples j.  Sample (j, k), attempt a:   */
+/* x = philox(counter (j, k, a, tag), key (seed lo, seed hi)); the point     */
+/* p_i = 2 u(x_i) - 1 (i < 3, u(x) = (x >> 8) 2^-24) is kept when            */
+/* 0 < norm2(p) <= 1 (:1178-1185); then r = u(x_3) and                       */
+/* d = nint(h (p / |p|) r), h_d = N_d - (N_d / 2 + 1) (:1187-1190), in FP32.  */
+/* d = 0 -> skip[0]++ (:1192-1195); |Q(d)| > q_range -> skip[1]++ (:1208-1210)*/
+/* else hist[nint(|Q| / q_step)][p] += Re(A_i A_j*) at d for pairs i >= j     */
+/* (:1201-1207; bins >= n_hist are dropped).  64 misses -> skip[0]++.        */
+/* amp [nsp][N3][N2][N1] complex (re, im), mode i_d - N_d / 2.  hist and     */
+/* skip are accumulated.                                                     */
+/* ------------------------------------------------------------------------ */
+int orc_sql_hist_mc(int nsp, const int64_t *N, const float *dq, const float *ainv, float q_range, float q_step,
+                    int n_hist, int64_t n_h, int64_t n_mc, uint64_t seed, uint32_t tag, const double *amp,
+                    double *hist, int64_t *skip)
+{
+    if (nsp < 1 || nsp > 8 || n_hist < 1 || n_h < 1 || n_mc < 1) return 2;
+    const int npairs = nsp * (nsp + 1) / 2;
+    const int64_t nm = N[0] * N[1] * N[2];
+    const int64_t total = n_h * n_mc;
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int c[3];
+    float h[3];
+    for (int d = 0; d < 3; ++d) {
+        c[d] = (int)(N[d] / 2);
+        h[d] = (float)(N[d] - (c[d] + 1));
+    }
+    const int nthr = orc_num_threads();
+    double *th = (double *)calloc((size_t)nthr * n_hist * npairs, sizeof(double));
+    int64_t *ts = (int64_t *)calloc((size_t)nthr * 2, sizeof(int64_t));
+#pragma omp parallel num_threads(nthr)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num();
+#else
+        const int tid = 0;
+#endif
+        double *hh = th + (size_t)tid * n_hist * npairs;
+        int64_t *ss = ts + 2 * tid;
+#pragma omp for schedule(static)
+        for (int64_t idx = 0; idx < total; ++idx) {
+            const uint32_t j = (uint32_t)(idx % n_mc), k = (uint32_t)(idx / n_mc);
+            uint32_t x[4];
+            float p[3], pn = 0.0f;
+            int ok = 0;
+            for (uint32_t a = 0; a < 64 && !ok; ++a) {
+                const uint32_t ctr[4] = {j, k, a, tag};
+                orc_philox4x32_10(ctr, key, x);
+                for (int i = 0; i < 3; ++i) p[i] = 2.0f * orc_unit(x[i]) - 1.0f;
+                pn = sqrtf((p[0] * p[0] + p[1] * p[1]) + p[2] * p[2]);
+                ok = pn > 0.0f && pn <= 1.0f;
+            }
+            if (!ok) {
+                ss[0]++;
+                continue;
+            }
+            const float r = orc_unit(x[3]);
+            int d[3];
+            for (int i = 0; i < 3; ++i) d[i] = (int)roundf((h[i] * (p[i] / pn)) * r);
+            if (d[0] == 0 && d[1] == 0 && d[2] == 0) {
+                ss[0]++;
+                continue;
+            }
+            int inside = 1;
+            for (int i = 0; i < 3; ++i) inside &= (c[i] + d[i] >= 0 && c[i] + d[i] < N[i]);
+            const float q = inside ? orc_sql_qnorm(dq, ainv, d[0], d[1], d[2]) : 0.0f;
+            if (!inside || !(q <= q_range)) {
+                ss[1]++;
+                continue;
+            }
+            const int b = (int)roundf(q / q_step);
+            if (b >= n_hist) continue;
+            const int64_t m = (c[0] + d[0]) + N[0] * ((c[1] + d[1]) + N[1] * (int64_t)(c[2] + d[2]));
+            int pp = 0;
+            for (int s = 0; s < nsp; ++s)
+                for (int t = 0; t <= s; ++t, ++pp) {
+                    const double *as = amp + 2 * (s * nm + m), *at = amp + 2 * (t * nm + m);
+                    hh[b * npairs + pp] += as[0] * at[0] + as[1] * at[1];
+                }
+        }
+    }
+    for (int t = 0; t < nthr; ++t) {
+        for (int64_t i = 0; i < (int64_t)n_hist * npairs; ++i) hist[i] += th[(size_t)t * n_hist * npairs + i];
+        skip[0] += ts[2 * t];
+        skip[1] += ts[2 * t + 1];
+    }
+    free(th);
+    free(ts);
     return 0;
 }
 

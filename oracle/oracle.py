@@ -60,6 +60,9 @@ def lib():
         L.orc_iqt.argtypes = [i64, i64, i32, vp, vp, vp, vp, vp, i64, i64, i64, i32, i32, dbl, vp]
         L.orc_speckle_pass.argtypes = [i64, vp, dbl]
         L.orc_dos.argtypes = [i64, i64, vp, vp, i32, i64, vp]
+        L.orc_philox4x32_10.argtypes = [vp, vp, vp]
+        L.orc_sql_hist_mc.argtypes = [i32, vp, vp, vp, C.c_float, C.c_float, i32, i64, i64, C.c_uint64, C.c_uint32,
+                                      vp, vp, vp]
         L.orc_oneph.argtypes = [i64, i32, vp, i32, i32, C.c_float, C.c_float, vp, vp, vp, vp]
         L.orc_speckle_pass.restype = None
         L.orc_es_spread.argtypes = [i32, i64, vp, vp, vp, vp, vp, i32, dbl, vp]
@@ -308,6 +311,33 @@ def dos(vel, at_ind, nsp, n_freq):
     if rc:
         raise RuntimeError(f"orc_dos rc={rc}")
     return out
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of one 128-bit counter under a 64-bit key (see orc_philox4x32_10)."""
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def sql_hist_mc(amp, n_modes, dq, a_cell_inv, q_range, q_step, n_hist, n_h, n_mc, seed, tag=0):
+    """mp_sql's Monte Carlo histogram of one frame (see orc_sql_hist_mc):
+    amp complex [nsp, N3, N2, N1] (or [nsp, nmodes]); returns (hist
+    [n_hist, npairs], (skip1, skip2))."""
+    amp = np.ascontiguousarray(amp, np.complex128)
+    nsp = amp.shape[0]
+    N = np.ascontiguousarray(n_modes, np.int64)
+    dq = np.ascontiguousarray(dq, np.float32)
+    ai = np.ascontiguousarray(a_cell_inv, np.float32).reshape(-1)
+    hist = np.zeros((int(n_hist), nsp * (nsp + 1) // 2))
+    skip = np.zeros(2, np.int64)
+    rc = lib().orc_sql_hist_mc(nsp, _p(N), _p(dq), _p(ai), float(q_range), float(q_step), int(n_hist), int(n_h),
+                               int(n_mc), int(seed), int(tag), _p(amp), _p(hist), _p(skip))
+    if rc:
+        raise RuntimeError(f"orc_sql_hist_mc rc={rc}")
+    return hist, (int(skip[0]), int(skip[1]))
 
 
 def es_params(eps: float, single: bool = False):
