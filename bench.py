@@ -737,10 +737,13 @@ def copy_calibration(mib):
 def sql_bench(a):
     """mp_sql on the reference's own Al92U08 snapshot (30,000 atoms, 79.8 A
     box): Q range 20 1/A = 509^3 modes per species, X-ray weights.  A step is
-    one snapshot: 2 species' 3D NUFFTs (FP64, eps 1e-6) + the |Q| histogram;
-    the output functions (host, 2001 bins) are outside the timed region.  The
-    CPU leg times the host restatement of the same steps (ES spreading in C,
-    scipy FFT, numpy binning) at Q range 5 1/A (129^3 modes) as its sample."""
+    one snapshot: 2 species' 3D NUFFTs (eps 1e-6) + the |Q| histogram; the
+    output functions (host, 2001 bins) are outside the timed region.  The
+    value is the FP32 form (the reference's amplitudes are REAL(4) complex;
+    FP64 histogram sums), with the FP64 form and the Monte Carlo integration
+    (16 x 10^6 samples per snapshot, FP32) beside it.  The CPU leg times the
+    host restatement of the same steps (ES spreading in C, scipy FFT, numpy
+    binning) at Q range 5 1/A (129^3 modes) as its sample."""
     import torch
 
     from mp_tools_amd import mpdat, sql
@@ -749,14 +752,19 @@ def sql_bench(a):
     counts = snap.species_counts
     g = sql.geometry(snap.a_par, snap.a_cell_inv, snap.header["N_ROW"], snap.n_tot, 20.0, 0.01)
     torch.cuda.set_device(0)
-    for _ in range(max(1, a.warmup)):
-        sql.histogram([snap.at_pos], counts, g)
-    tm = {}
     frames = [snap.at_pos] * a.steps
-    t0 = time.time()
-    h = sql.histogram(frames, counts, g, timing=tm)
-    wall = time.time() - t0
-    ms = tm["ms"] / a.steps
+
+    def timed(precision, **kw):
+        for _ in range(max(1, a.warmup)):
+            sql.histogram([snap.at_pos], counts, g, precision=precision, **kw)
+        tm = {}
+        t0 = time.time()
+        sql.histogram(frames, counts, g, precision=precision, timing=tm, **kw)
+        return tm["ms"] / a.steps, time.time() - t0, tm
+
+    ms, wall, _ = timed("fp32")
+    ms64, _, _ = timed("fp64")
+    ms_mc, _, tmc = timed("fp32", integration="mc", n_h=16, seed=1)
     # host restatement on a bounded sample: Q range 5
     gs = sql.geometry(snap.a_par, snap.a_cell_inv, snap.header["N_ROW"], snap.n_tot, 5.0, 0.01)
     cores = os.cpu_count() or 1
@@ -778,9 +786,13 @@ def sql_bench(a):
             np.bincount(b, (amps[i_][sel] * np.conj(amps[j_][sel])).real, minlength=gs.n_hist)
     cpu_s = time.time() - t1
     res = {
-        "metric": "mp_sql snapshots/s (Al92U08, 30,000 atoms, Q range 20/A = 509^3 modes x 2 species, FP64 NUFFT + |Q| histogram)",
+        "metric": "mp_sql snapshots/s (Al92U08, 30,000 atoms, Q range 20/A = 509^3 modes x 2 species, FP32 NUFFT + |Q| histogram)",
         "value": 1000.0 / ms, "unit": "snapshots/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "fp64_form": {"value": 1000.0 / ms64, "ms_per_step": ms64},
+        "monte_carlo": {"value": 1000.0 / ms_mc, "ms_per_step": ms_mc, "samples_per_snapshot": 16 * sql.N_MC,
+                        "skips_last_snapshot": list(tmc["skips"]),
+                        "note": "Gauss sum replaced by 16 chains of 10^6 Philox-drawn modes (mp_sql j_acc = 2)"},
         "data": "the reference's example snapshot examples/amorphous_Al92U08/data/Al_U_large_n0001.dat (tests/golden/al_u)",
         "config": {"workload": "mp_sql Al/U Q_range 20, q_step 0.01, X-ray", "n_qq": list(g.n_qq), "n_hist": g.n_hist},
         "wall_s": wall,

@@ -59,25 +59,37 @@ def test_sql_hist_kernel_vs_numpy(gpu_lib):
     assert rc == _lib.ERR_SIZE
 
 
-@pytest.fixture(scope="module")
-def al_u_hist():
+def _al_u(precision):
     from mp_tools_amd import mpdat, sql
     s = mpdat.read_dat(os.path.join(HERE, "golden", "al_u", "Al_U_large_n0001.dat"))
     g = sql.geometry(s.a_par, s.a_cell_inv, s.header["N_ROW"], s.n_tot, 20.0, 0.01)
     tm = {}
-    h = sql.histogram([s.at_pos], s.species_counts, g, eps=1e-6, precision="fp64", timing=tm)
-    print(f"mp_sql Al/U 509^3: {tm['ms']:.1f} ms per snapshot (2 species)")
+    h = sql.histogram([s.at_pos], s.species_counts, g, eps=1e-6, precision=precision, timing=tm)
+    print(f"mp_sql Al/U 509^3 {precision}: {tm['ms']:.1f} ms per snapshot (2 species)")
     return s, g, h
+
+
+@pytest.fixture(scope="module")
+def al_u_hist():
+    return _al_u("fp64")
+
+
+@pytest.fixture(scope="module")
+def al_u_hist32():
+    """The FP32 form: FP32 NUFFT (the reference's own amplitudes are REAL(4)
+    complex), FP64 histogram sums."""
+    return _al_u("fp32")
 
 
 # relative l2 (over the written rows) of each column against the reference
 GOLDEN_TOL = 2e-3
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
 @pytest.mark.parametrize("key,mode", [("Z", 5), ("I", 6), ("S", 4), ("Iu", 7)])
-def test_al_u_against_reference_tables(gpu_lib, al_u_hist, key, mode):
+def test_al_u_against_reference_tables(gpu_lib, request, precision, key, mode):
     from mp_tools_amd import sql
-    s, g, h = al_u_hist
+    s, g, h = request.getfixturevalue("al_u_hist" if precision == "fp64" else "al_u_hist32")
     gold = np.load(os.path.join(HERE, "golden", "al_u", "al_u_sql_golden.npz"))
     ref = gold[key]
     q, tot, parts = sql.output(h, g, mode, s.occupancies, "xray", ffpar=gold["ffpar"],
