@@ -324,7 +324,8 @@ def _timed(fn, reps=3):
 
 def next_rows_record(a, dev, eng, amp, pos, counts, nat, dt):
     """Times of the paths either side of the S(Q,w) map on the same workload:
-    an E(Q) cut and I(Q,t) (map at 4 lags, cut at every lag) from the C2
+    an E(Q) cut and I(Q,t) (map at 4 lags by direct sums and at every lag by
+    FFT correlation, cut at every lag) from the C2
     amplitudes, the one-phonon amplitudes of the C2 lattice (5 sites x
     40^3 cells, 64 frames) and the mp_dos spectra of its velocities (central
     differences of the positions, all frames).  Algorithmic bytes: amplitudes / positions /
@@ -350,6 +351,10 @@ def next_rows_record(a, dev, eng, amp, pos, counts, nat, dt):
     ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, kind=1, m_lo=0, n_f=4, nat=nat), reps=2)
     rec["iqt_map_4_lags"] = {"modes": NQ, "n_int": n_i, "ms": ms,
                              "alg_GBps": (T * NQ * 8 + 5 * NQ * 8) / (ms * 1e-3) / 1e9}
+    ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, kind=1, nat=nat), reps=2)
+    rec["iqt_map_all_lags"] = {"modes": NQ, "n_int": n_i, "lags": n_lag, "ms": ms, "method": "FFT correlation",
+                               "alg_GBps": (T * NQ * 8 + (n_lag + 1) * NQ * 8) / (ms * 1e-3) / 1e9}
+    torch.cuda.empty_cache()
     # one-phonon amplitudes of the lattice (CELL layout: 5 site blocks)
     R, _ = synth.perovskite_lattice(40)
     cells = np.floor(R + 20.0 + 1e-9).astype(np.int32)

@@ -199,6 +199,11 @@ static std::vector<double> deconv_factors(int64_t N, int64_t n, int w, double be
 
 template <typename R> using Cx = typename cplx<R>::T;
 
+// I(Q,t) by FFT correlation from this many lags on (below it the direct
+// sums read less), with blocks sized to keep two per CU's LDS
+constexpr int IQT_FFT_MIN_LAGS = 16;
+constexpr size_t IQT_FFT_LDS_TARGET = 76 * 1024;
+
 template <typename R> static void upload_twiddles(DevBuf &buf, int n, hipStream_t st, std::vector<Cx<R>> &host)
 {
     host.resize((size_t)n);
@@ -960,6 +965,10 @@ template <typename R> struct Engine {
     std::vector<Cx<R>> twT_h;
     int win_n = -1, win_kind = -1, twT_n = -1;
     RadixPlan rpT{};
+    DevBuf twI;                     // FP64 twiddles of the I(Q,t) correlation
+    std::vector<Cx<double>> twI_h;
+    int twI_n = -1;
+    RadixPlan rpI{};
 
     int64_t nq() const { return core.N[0] * core.N[1] * core.N[2]; }
     int64_t amp_frame_elems() const { return combine ? nq() : nq() * nact; }
@@ -1145,8 +1154,28 @@ template <typename R> struct Engine {
         p.norm_tot = 1.0 / ((double)T * r->nat * r->nat);
         const Cx<R> *amp = static_cast<const Cx<R> *>(d_amp);
         if (p.kind != 0) {
+            // many lags: FFT correlation (three FP64 LDS transforms per Q);
+            // a few lags, or a T without a radix plan / LDS room: direct sums
+            RadixPlan rp{};
+            const bool fft = p.n_f >= IQT_FFT_MIN_LAGS && T <= iqt_fft_tile_elems() &&
+                             iqt_fft_lds(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
             const int ev = tm.begin(ST_REDUCE, st);
-            launch_iqt<R>(p, amp, r->d_qfac, d_S, d_Stot, st);
+            if (fft) {
+                if (twI_n != (int)T) {
+                    upload_twiddles<double>(twI, (int)T, st, twI_h);
+                    twI_n = (int)T;
+                    rpI = rp;
+                }
+                // columns per block: the most that keep two blocks per CU
+                // (or one, for long series)
+                int cols = 1;
+                while (2 * cols * T <= iqt_fft_tile_elems() && 2 * cols <= SQW_MAX_COLS &&
+                       iqt_fft_lds(T, 2 * cols) <= IQT_FFT_LDS_TARGET)
+                    cols *= 2;
+                launch_iqt_fft<R>(p, rpI, cols, amp, twI.as<Cx<double>>(), r->d_qfac, d_S, d_Stot, st);
+            } else {
+                launch_iqt<R>(p, amp, r->d_qfac, d_S, d_Stot, st);
+            }
             tm.end(ev, st);
             HIPCHK(hipGetLastError());
             if (!sync) return;

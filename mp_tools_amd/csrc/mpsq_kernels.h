@@ -280,6 +280,14 @@ void launch_cols_to_f64(const float *in, int64_t T, int64_t ncs, int64_t c0, int
 template <typename R>
 void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,
                 hipStream_t st);
+// the same sums by FFT correlation in LDS (FP64): cols Q columns per block,
+// T * cols <= iqt_fft_tile_elems(), iqt_fft_lds(T, cols) <= SQW_MAX_LDS,
+// rp the radix plan of T, tw the FP64 twiddles e^{-2 pi i m / T}
+int iqt_fft_tile_elems();
+size_t iqt_fft_lds(int64_t T, int cols);
+template <typename R>
+void launch_iqt_fft(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,
+                    const double2 *tw, const float *qfac, double *S, double *Stot, hipStream_t st);
 
 // One-phonon amplitudes (oneph_kernels.hip)
 constexpr int ONEPH_MAX_SP = 64;

@@ -17,6 +17,11 @@
 
 namespace mpsq {
 
+// LDS tiles of the time-FFT kernels: E elements per thread, NT threads
+template <typename R> struct SqwTiles;
+template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
+template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };
+
 // amplitude column of output q; -1 for an index list entry outside
 // [0, NQs), whose outputs are written as NaN (nothing is read for it)
 __device__ __forceinline__ int64_t src_q(const SqwParams &p, int64_t q)
@@ -388,6 +393,135 @@ __global__ __launch_bounds__(256) void k_iqt(SqwParams p, const typename cplx<R>
     }
 }
 
+// I(Q,t) at many lags by FFT correlation (the sums of k_iqt, computed in
+// O(T log T) per Q instead of O(n_f n_int)).  With x(t) = C'(t), t < T, and
+// y(t) = C'(t) for t < n_int, 0 beyond, the circular correlation over T
+// points is
+//   sum_{s < n_int} x((s + l) mod T) conj(y(s)) = IFFT(X conj(Y))[l],
+// which equals k_iqt's sum for 0 <= l <= T - n_int (s + l never wraps).
+// The inverse transform is the forward one of the conjugate,
+// IFFT(Z) = conj(FFT(conj(Z))) / T, and conj(X conj(Y)) = conj(X) Y, so
+//   I(l) = Re FFT(conj(X) Y)[l] / T.
+// A block holds `cols` Q columns of x and y in LDS (FP64 whatever the
+// amplitudes' precision, like k_iqt's sums); mean and sum |C|^2 per column as
+// in k_sqw_single.
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_iqt_fft(SqwParams p, RadixPlan rp, int cols, FDiv dcols,
+                                                const typename cplx<R>::T *__restrict__ amp,
+                                                const double2 *__restrict__ tw, const float *__restrict__ qfac,
+                                                double *__restrict__ S, double *__restrict__ Stot)
+{
+    using A = typename cplx<R>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int T = (int)p.T;
+    double2 *xt = reinterpret_cast<double2 *>(smem_raw);  // [T][cols]
+    double2 *yt = xt + T * cols;                           // [T][cols]
+    double2 *tw_l = yt + T * cols;                         // [T]
+    double *red = reinterpret_cast<double *>(tw_l + T);    // [NT / 64][cols][3]
+    __shared__ double mean_r[SQW_MAX_COLS], mean_i[SQW_MAX_COLS], qf_s[SQW_MAX_COLS];
+    for (int i = threadIdx.x; i < T; i += NT) tw_l[i] = tw[i];
+    const int64_t q0 = (int64_t)blockIdx.x * cols;
+    const int ncol = (int)min((int64_t)cols, p.NQ - q0);
+    {
+        A v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
+            const int64_t s = (i < T * cols && c < ncol) ? src_q(p, q0 + c) : -1;
+            v[k] = (s >= 0) ? amp[(int64_t)t * p.NQs + s] : cmk<A>((R)0, (R)0);
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < T * cols) xt[i] = make_double2((double)v[k].x, (double)v[k].y);
+        }
+    }
+    __syncthreads();
+    {
+        const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
+        double sr = 0.0, si = 0.0, ss = 0.0;
+        for (int t = seg; t < T; t += nseg) {
+            const double2 v = xt[t * cols + c];
+            sr += v.x;
+            si += v.y;
+            ss += v.x * v.x + v.y * v.y;
+        }
+        for (int o = cols; o < 64; o <<= 1) {
+            sr += __shfl_xor(sr, o, 64);
+            si += __shfl_xor(si, o, 64);
+            ss += __shfl_xor(ss, o, 64);
+        }
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        if (lane < cols) {
+            red[(wv * cols + lane) * 3 + 0] = sr;
+            red[(wv * cols + lane) * 3 + 1] = si;
+            red[(wv * cols + lane) * 3 + 2] = ss;
+        }
+        __syncthreads();
+        if (threadIdx.x < cols) {
+            double a = 0.0, b = 0.0, q = 0.0;
+            for (int k = 0; k < NT / 64; ++k) {
+                a += red[(k * cols + threadIdx.x) * 3 + 0];
+                b += red[(k * cols + threadIdx.x) * 3 + 1];
+                q += red[(k * cols + threadIdx.x) * 3 + 2];
+            }
+            mean_r[threadIdx.x] = p.subtract_mean ? a / (double)T : 0.0;
+            mean_i[threadIdx.x] = p.subtract_mean ? b / (double)T : 0.0;
+            const int64_t s = threadIdx.x < ncol ? src_q(p, q0 + threadIdx.x) : -1;
+            qf_s[threadIdx.x] = (qfac && s >= 0) ? (double)qfac[s] : 1.0;
+            if (Stot && threadIdx.x < ncol) Stot[q0 + threadIdx.x] = s >= 0 ? q * p.norm_tot : qnan();
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < T * cols; i += NT) {
+        const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
+        const double2 v = xt[i];
+        const double2 x = make_double2((v.x - mean_r[c]) * qf_s[c], (v.y - mean_i[c]) * qf_s[c]);
+        xt[i] = x;
+        yt[i] = t < p.n_int ? x : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    lds_fft<double2, E>(xt, rp, cols, dcols, cols, 1, tw_l, true);
+    lds_fft<double2, E>(yt, rp, cols, dcols, cols, 1, tw_l, true);
+    for (int i = threadIdx.x; i < T * cols; i += NT) {
+        const double2 a = xt[i], b = yt[i];
+        xt[i] = make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);  // conj(a) b
+    }
+    __syncthreads();
+    lds_fft<double2, E>(xt, rp, cols, dcols, cols, 1, tw_l, true);
+    const double scale = p.norm_s / (double)T;
+    for (int i = threadIdx.x; i < p.n_f * cols; i += NT) {
+        const int l = (int)fdiv((uint32_t)i, dcols), c = i - l * cols;
+        if (c >= ncol) continue;
+        S[(int64_t)l * p.NQ + q0 + c] = src_q(p, q0 + c) >= 0 ? xt[(int)(p.m_lo + l) * cols + c].x * scale : qnan();
+    }
+}
+
+int iqt_fft_tile_elems() { return SqwTiles<double>::E * SqwTiles<double>::NT; }
+
+size_t iqt_fft_lds(int64_t T, int cols)
+{
+    return (size_t)T * (size_t)(2 * cols + 1) * sizeof(double2) +
+           sizeof(double) * 3 * (SqwTiles<double>::NT / 64) * (size_t)cols;
+}
+
+template <typename R>
+void launch_iqt_fft(const SqwParams &p, const RadixPlan &rp, int cols, const typename cplx<R>::T *amp,
+                    const double2 *tw, const float *qfac, double *S, double *Stot, hipStream_t st)
+{
+    constexpr int E = SqwTiles<double>::E, NT = SqwTiles<double>::NT;
+    const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_iqt_fft<R, E, NT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_iqt_fft<R, E, NT>), dim3(nb), dim3(NT), iqt_fft_lds(p.T, cols), st, p, rp, cols,
+                       make_fdiv((uint32_t)cols), amp, tw, qfac, S, Stot);
+}
+
 template <typename R>
 void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,
                 hipStream_t st)
@@ -453,9 +587,6 @@ void launch_cols_to_f64(const float *in, int64_t T, int64_t ncs, int64_t c0, int
                        reinterpret_cast<const float2 *>(in), T, ncs, c0, nc, reinterpret_cast<double2 *>(out));
 }
 
-template <typename R> struct SqwTiles;
-template <> struct SqwTiles<float> { static constexpr int E = 16, NT = 512; };
-template <> struct SqwTiles<double> { static constexpr int E = 8, NT = 512; };
 
 template <typename R> int sqw_tile_elems() { return SqwTiles<R>::E * SqwTiles<R>::NT; }
 
@@ -501,7 +632,9 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
                                        hipStream_t);                                                                  \
     template void launch_sqw_sliding<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *,    \
                                         hipStream_t);                                                                 \
-    template void launch_iqt<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *, hipStream_t);
+    template void launch_iqt<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *, hipStream_t); \
+    template void launch_iqt_fft<R>(const SqwParams &, const RadixPlan &, int, const cplx<R>::T *, const double2 *,  \
+                                    const float *, double *, double *, hipStream_t);
 
 MPSQ_SQW_INST(float)
 MPSQ_SQW_INST(double)

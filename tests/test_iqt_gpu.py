@@ -1,6 +1,7 @@
 """GPU parity of the reduction's other outputs against the FP64 oracle:
 the intermediate function I(Q,t) (mp_sqom56.f90:1467-1471 map form,
-:1504-1510 cut form, normalised by nat^2 at :1574-1575), the E(Q) /
+:1504-1510 cut form, normalised by nat^2 at :1574-1575; direct sums below
+16 lags, FFT correlation from 16 on), the E(Q) /
 I(Q,t) cuts through a mode index list (:1480-1510: the same per-Q
 arithmetic as the map, at the pixels of the cut) and the reference's
 window-length rule (:683-686).
@@ -37,6 +38,11 @@ IQT_CASES = [
     (64, 32, 1, 5, 10, False, True),       # a lag window, no mean subtraction
     (33, 1, 2, 0, 33, True, False),        # n_int = 1
     (512, 256, 1, 0, 257, True, False),    # the reference's default n_int = nfile / 2 at 512 frames
+    # 16 lags or more take the FFT correlation (T with a radix plan)
+    (48, 7, 2, 3, 30, True, True),         # a lag offset, per-Q factor
+    (100, 50, 1, 0, 51, False, False),     # T = 4 x 5 x 5, no mean subtraction
+    (2048, 1024, 1, 0, 17, True, False),   # one column per block
+    (3000, 1500, 2, 1480, 20, True, True), # T = 8 x 3 x 5^3, the last lags
 ]
 
 
@@ -60,6 +66,24 @@ def test_iqt_vs_oracle(gpu_lib, precision, T, n_int, kind, m_lo, n_f, sub, useq)
     assert I.shape == (n_f, NQ)
     assert rel_l2(I.cpu().numpy(), Ir) < 1e-12
     assert rel_l2(St.cpu().numpy(), Str) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_fft_correlation_equals_direct_sums(gpu_lib, precision):
+    """16 lags take the FFT correlation, 15 the direct sums: on the same
+    amplitudes the shared lags agree to 1e-13 (FP64 either way)."""
+    import torch
+    from mp_tools_amd.sqw import SqwEngine
+    T, n_int, nm = 128, 64, (16, 12)
+    _, damp = _amps(T, 192, 17, precision)
+    qf = torch.rand(192, device="cuda") + 0.5
+    eng = SqwEngine(2, nm, [10, 10], precision=precision)
+    for kind in (1, 2):
+        a, ta = eng.reduce(damp, n_int=n_int, nat=NAT, m_lo=7, n_f=16, kind=kind, qfac=qf)
+        b, tb = eng.reduce(damp, n_int=n_int, nat=NAT, m_lo=7, n_f=15, kind=kind, qfac=qf)
+        torch.cuda.synchronize()
+        assert rel_l2(a[:15].cpu().numpy(), b.cpu().numpy()) < 1e-13
+        assert rel_l2(ta.cpu().numpy(), tb.cpu().numpy()) < 1e-14
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
