@@ -13,8 +13,9 @@
 //
 // Each workgroup sums its share of the modes into an LDS histogram (FP64
 // LDS atomics; consecutive modes of a row fall in different bins except
-// near the axes) and writes it to its own slice of a partial buffer; a
-// second kernel adds the slices in a fixed order.
+// near the axes) and writes it to its own slice of a partial buffer; the
+// slices are added in a fixed order, by groups over many blocks and then the
+// group sums.
 #include <algorithm>
 #include <mutex>
 
@@ -79,6 +80,33 @@ __global__ void k_sql_reduce(const double *__restrict__ part, int nparts, int nb
     out[i] += s;
 }
 
+// part2[grp][i] = sum of part[k][i] over the slices k of group grp
+// (k = grp * per .. min((grp + 1) * per, nparts) - 1, in order): the first of
+// two levels, so that the slices are added by many blocks at once
+__global__ void k_sql_reduce_groups(const double *__restrict__ part, int nparts, int per, int nb,
+                                    double *__restrict__ part2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    const int k0 = blockIdx.y * per, k1 = min(k0 + per, nparts);
+    double s = 0.0;
+    for (int k = k0; k < k1; ++k) s += part[(int64_t)k * nb + i];
+    part2[(int64_t)blockIdx.y * nb + i] = s;
+}
+
+constexpr int SQL_REDUCE_GROUPS = 32;
+
+// out[i] += the nparts slices of part, through SQL_REDUCE_GROUPS group sums
+// in part2 (fixed order: the result does not depend on timing)
+static void reduce_slices(const double *part, int nparts, int nb, double *part2, double *out, hipStream_t st)
+{
+    const int per = (nparts + SQL_REDUCE_GROUPS - 1) / SQL_REDUCE_GROUPS;
+    const int ng = (nparts + per - 1) / per;
+    hipLaunchKernelGGL(k_sql_reduce_groups, dim3((unsigned)((nb + 255) / 256), (unsigned)ng), dim3(256), 0, st, part,
+                       nparts, per, nb, part2);
+    hipLaunchKernelGGL(k_sql_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, part2, ng, nb, out);
+}
+
 // Returns 0, or -1 when the histogram does not fit one workgroup's LDS.
 int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, hipStream_t st)
 {
@@ -90,7 +118,8 @@ int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, h
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int nparts = std::max(1, ncu) * 2;
     double *part = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * (size_t)nparts * nb, st) != hipSuccess)
+    if (hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * (size_t)(nparts + SQL_REDUCE_GROUPS) * nb,
+                       st) != hipSuccess)
         return -2;
     if (fp64)
         hipLaunchKernelGGL((k_sql_hist<double2>), dim3(nparts), dim3(256), lds, st,
@@ -98,7 +127,7 @@ int launch_sql_hist(const void *amp, bool fp64, const SqlGeom &g, double *out, h
     else
         hipLaunchKernelGGL((k_sql_hist<float2>), dim3(nparts), dim3(256), lds, st,
                            static_cast<const float2 *>(amp), g, part);
-    hipLaunchKernelGGL(k_sql_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, part, nparts, nb, out);
+    reduce_slices(part, nparts, nb, part + (size_t)nparts * nb, out, st);
     (void)hipFreeAsync(part, st);
     return 0;
 }
@@ -259,7 +288,8 @@ int launch_sql_hist_mc(const void *amp, bool fp64, const SqlGeom &g, const SqlMc
     const uint64_t total = (uint64_t)mc.n_h * (uint64_t)mc.n_mc;
     const int nparts = (int)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)std::max(1, ncu) * 4, (total + 255) / 256));
     double *part = nullptr;
-    if (hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * (size_t)nparts * nb, st) != hipSuccess)
+    if (hipMallocAsync(reinterpret_cast<void **>(&part), sizeof(double) * (size_t)(nparts + SQL_REDUCE_GROUPS) * nb,
+                       st) != hipSuccess)
         return -2;
     if (fp64)
         hipLaunchKernelGGL((k_sql_hist_mc<double2>), dim3(nparts), dim3(256), lds, st,
@@ -267,7 +297,7 @@ int launch_sql_hist_mc(const void *amp, bool fp64, const SqlGeom &g, const SqlMc
     else
         hipLaunchKernelGGL((k_sql_hist_mc<float2>), dim3(nparts), dim3(256), lds, st,
                            static_cast<const float2 *>(amp), g, mc, part, skip);
-    hipLaunchKernelGGL(k_sql_reduce, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, part, nparts, nb, out);
+    reduce_slices(part, nparts, nb, part + (size_t)nparts * nb, out, st);
     (void)hipFreeAsync(part, st);
     return 0;
 }
