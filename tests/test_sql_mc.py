@@ -152,6 +152,19 @@ def test_chain_limit_and_normalisation():
     assert np.allclose(out[:, :, 1:], 0.1 / (np.float32(0.1) * 2) * np.float32(0.4) / 750)
 
 
+def test_histogram_argument_checks():
+    """Before any device work: the integration name, n_h >= 1 and the
+    reference's chain limit for 10^6-sample chains (mp_sql56.f90:1069)."""
+    geo = sql.SqlGeometry(np.float32([0.1] * 3), np.eye(3, dtype=np.float32), (509, 509, 509), np.float32(20.0),
+                          np.float32(0.01), 2001, 1.0)
+    with pytest.raises(ValueError):
+        sql.histogram([], [1], geo, integration="gaus")
+    with pytest.raises(ValueError):
+        sql.histogram([], [1], geo, integration="mc", n_h=0)
+    with pytest.raises(ValueError):
+        sql.histogram([], [1], geo, integration="mc", n_h=66)
+
+
 # ---------------------------------------------------------------------------
 # GPU: the HIP kernel through the C ABI against the oracle
 # ---------------------------------------------------------------------------
