@@ -2021,10 +2021,13 @@ struct mpsq_sqw_s {
     }
 };
 
-// mp_dos spectra: the velocity columns' windowed time FFT through the
-// S(Q,w) reduction kernels (one window over all frames, no mean subtraction,
-// Hann window of the (i + 1) / (n + 1) form, norm 1 / T), then the species
-// sums; chunks of columns keep the |FFT|^2 rows within 512 MB.
+// mp_dos spectra: the velocity columns' windowed time FFT (one window over
+// all frames, no mean subtraction, Hann window of the (i + 1) / (n + 1) form,
+// norm 1 / T) and the species sums.  With a radix plan for T, one fused
+// kernel per group of blocks (FFT in LDS, per-block species sums) and a
+// fixed-order reduction of the blocks' sums; otherwise the S(Q,w) sliding
+// kernel into |FFT|^2 rows (chunks of columns within 512 MB) and
+// k_dos_species.
 template <typename R>
 static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t *d_at_ind, int nsp, int64_t n_freq,
                     double *d_dos, hipStream_t st)
@@ -2043,7 +2046,7 @@ static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t 
     p.norm_s = 1.0 / (double)T;
     p.norm_tot = 0.0;
     RadixPlan rp{};
-    const bool single = (T <= sqw_tile_elems<R>()) && sqw_single_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
+    const bool single = (T <= sqw_tile_elems<R>()) && dos_fused_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
     DevBuf tw, win, S, buf;
     int cols = 1;
     if (single) {
@@ -2056,13 +2059,39 @@ static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t 
         HIPCHK(hipMemcpyAsync(win.p, wh.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS &&
-               sqw_single_lds<R>(T, 2 * cols) <= SQW_MAX_LDS)
+               dos_fused_lds<R>(T, 2 * cols) <= SQW_MAX_LDS)
             cols *= 2;
+    }
+    launch_zero_i32(reinterpret_cast<int *>(d_dos), (int)(2 * n_freq * nsp), st);
+    if (single) {
+        // one pass: FFT, |X|^2 and per-block species sums (no |FFT|^2 rows
+        // in HBM); blocks per launch keep their sums within 256 MB
+        const int64_t nout = (int64_t)nsp * n_freq;
+        const int64_t nblk = (ncs + cols - 1) / cols;
+        const int64_t bpl = std::max<int64_t>(1, std::min<int64_t>(nblk, ((int64_t)256 << 20) / (8 * nout)));
+        DevBuf part;
+        part.ensure(sizeof(double) * (size_t)((bpl + dos_reduce_groups()) * nout));
+        DosParams dp{};
+        dp.T = T;
+        dp.ncs = ncs;
+        dp.n_freq = n_freq;
+        dp.nsp = nsp;
+        dp.norm = 1.0 / (double)T;
+        for (int64_t b0 = 0; b0 < nblk; b0 += bpl) {
+            const int64_t nb = std::min(bpl, nblk - b0);
+            dp.c0 = b0 * cols;
+            dp.nc = std::min(nb * cols, ncs - dp.c0);
+            launch_dos_fused<R>(dp, rp, cols, d_vel, d_at_ind, tw.as<Cx<R>>(), win.as<double>(), part.as<double>(),
+                                st);
+            dos_reduce_parts(part.as<double>(), nb, nsp, (int)n_freq, part.as<double>() + bpl * nout, d_dos, st);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+        return;
     }
     const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(ncs, ((int64_t)512 << 20) / (8 * n_f)));
     S.ensure(sizeof(double) * (size_t)(n_f * chunk));
     if (sizeof(R) == 8) buf.ensure(sizeof(double2) * (size_t)(T * chunk));
-    launch_zero_i32(reinterpret_cast<int *>(d_dos), (int)(2 * n_freq * nsp), st);
     for (int64_t c0 = 0; c0 < ncs; c0 += chunk) {
         const int64_t nc = std::min(chunk, ncs - c0);
         p.NQ = nc;

@@ -276,6 +276,22 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
 void launch_dos_species(const double *S, int64_t ncols, int64_t c0, int64_t n_freq, const int *at_ind, int nsp,
                         double *dos, hipStream_t st);
 void launch_cols_to_f64(const float *in, int64_t T, int64_t ncs, int64_t c0, int64_t nc, double *out, hipStream_t st);
+// mp_dos spectra in one pass: time FFT of packed velocity columns
+// [c0, c0 + nc) (ncs per frame) and per-block species sums part[block][s][f];
+// dos_reduce_parts adds nrows such rows into dos[f][s] in a fixed order
+// (part2: dos_reduce_groups() * nsp * n_freq doubles)
+struct DosParams {
+    int64_t T, ncs, c0, nc, n_freq;
+    int nsp;
+    double norm;
+};
+template <typename R> size_t dos_fused_lds(int64_t T, int cols);
+template <typename R>
+void launch_dos_fused(const DosParams &p, const RadixPlan &rp, int cols, const float *vel, const int *at_ind,
+                      const typename cplx<R>::T *tw, const double *win, double *part, hipStream_t st);
+int dos_reduce_groups();
+void dos_reduce_parts(const double *part, int64_t nrows, int nsp, int n_freq, double *part2, double *dos,
+                      hipStream_t st);
 // I(Q,t): S[l][q] = norm Re sum_{i<n_int} C'(i + m_lo + l) conj C'(i), and S_tot
 template <typename R>
 void launch_iqt(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S, double *Stot,

@@ -569,6 +569,125 @@ void launch_dos_species(const double *S, int64_t ncols, int64_t c0, int64_t n_fr
                        reinterpret_cast<const int4 *>(at_ind), nsp, dos);
 }
 
+// The same spectra without the |FFT|^2 rows in HBM: a block loads `cols`
+// packed velocity columns (first column c0 + block * cols), applies the
+// window, runs the time FFT in LDS like k_sqw_single, and for every atom type
+// s and frequency f writes
+//   part[block][s][f] = sum over its columns of type s, in column order, of
+//                       (|X(f)|^2 + |X(-f)|^2) / 2 * norm
+// (FP64); dos_reduce_parts adds the blocks' rows in a fixed order.
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_dos_fused(DosParams p, RadixPlan rp, int cols, FDiv dcols,
+                                                  const float2 *__restrict__ vel, const int4 *__restrict__ at_ind,
+                                                  const typename cplx<R>::T *__restrict__ tw,
+                                                  const double *__restrict__ win, double *__restrict__ part)
+{
+    using C = typename cplx<R>::T;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int T = (int)p.T;
+    C *tile = reinterpret_cast<C *>(smem_raw);  // [T][cols]
+    C *tw_l = tile + T * cols;
+    __shared__ int spc[SQW_MAX_COLS];
+    for (int i = threadIdx.x; i < T; i += NT) tw_l[i] = tw[i];
+    const int64_t q0 = p.c0 + (int64_t)blockIdx.x * cols;
+    const int ncol = (int)min((int64_t)cols, p.c0 + p.nc - q0);
+    if ((int)threadIdx.x < cols) spc[threadIdx.x] = (int)threadIdx.x < ncol ? at_ind[(q0 + threadIdx.x) >> 1].w - 1 : -1;
+    {
+        C v[E];
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
+            v[k] = cmk<C>((R)0, (R)0);
+            if (i < T * cols && c < ncol) {
+                const float2 a = vel[(int64_t)t * p.ncs + q0 + c];
+                const double w = win[t];
+                v[k] = cmk<C>((R)((double)a.x * w), (R)((double)a.y * w));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int i = threadIdx.x + k * NT;
+            if (i < T * cols) tile[i] = v[k];
+        }
+    }
+    __syncthreads();
+    lds_fft<C, E>(tile, rp, cols, dcols, cols, 1, tw_l, true);
+    const int nf = (int)p.n_freq, nout = p.nsp * nf;
+    double *dst = part + (int64_t)blockIdx.x * nout;
+    for (int i = threadIdx.x; i < nout; i += NT) {
+        const int s = i / nf, f = i - s * nf;
+        const int fm = f == 0 ? 0 : T - f;
+        double acc = 0.0;
+        for (int c = 0; c < ncol; ++c)
+            if (spc[c] == s) {
+                const C a = tile[f * cols + c], b = tile[fm * cols + c];
+                acc += 0.5 * (((double)a.x * (double)a.x + (double)a.y * (double)a.y) * p.norm +
+                              ((double)b.x * (double)b.x + (double)b.y * (double)b.y) * p.norm);
+            }
+        dst[i] = acc;
+    }
+}
+
+template <typename R> size_t dos_fused_lds(int64_t T, int cols)
+{
+    return (size_t)T * (size_t)(cols + 1) * sizeof(typename cplx<R>::T);
+}
+
+template <typename R>
+void launch_dos_fused(const DosParams &p, const RadixPlan &rp, int cols, const float *vel, const int *at_ind,
+                      const typename cplx<R>::T *tw, const double *win, double *part, hipStream_t st)
+{
+    constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
+    const unsigned nb = (unsigned)((p.nc + cols - 1) / cols);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dos_fused<R, E, NT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_dos_fused<R, E, NT>), dim3(nb), dim3(NT), dos_fused_lds<R>(p.T, cols), st, p, rp, cols,
+                       make_fdiv((uint32_t)cols), reinterpret_cast<const float2 *>(vel),
+                       reinterpret_cast<const int4 *>(at_ind), tw, win, part);
+}
+
+// part2[g][i] = sum over the rows r = g * per .. of part[r][i], in order
+__global__ void k_dos_groups(const double *__restrict__ part, int64_t nrows, int64_t per, int nout,
+                             double *__restrict__ part2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nout) return;
+    const int64_t r0 = (int64_t)blockIdx.y * per, r1 = min(r0 + per, nrows);
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; ++r) s += part[r * nout + i];
+    part2[(int64_t)blockIdx.y * nout + i] = s;
+}
+
+// dos[f][s] += sum over g of part2[g][s * n_freq + f], in order
+__global__ void k_dos_final(const double *__restrict__ part2, int ng, int nsp, int n_freq, double *__restrict__ dos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nsp * n_freq) return;
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += part2[(int64_t)g * nsp * n_freq + i];
+    const int sp = i / n_freq, f = i - sp * n_freq;
+    dos[(int64_t)f * nsp + sp] += s;
+}
+
+int dos_reduce_groups() { return 64; }
+
+void dos_reduce_parts(const double *part, int64_t nrows, int nsp, int n_freq, double *part2, double *dos,
+                      hipStream_t st)
+{
+    const int nout = nsp * n_freq, G = dos_reduce_groups();
+    const int64_t per = (nrows + G - 1) / G;
+    const int ng = (int)((nrows + per - 1) / per);
+    hipLaunchKernelGGL(k_dos_groups, dim3((unsigned)((nout + 255) / 256), (unsigned)ng), dim3(256), 0, st, part, nrows,
+                       per, nout, part2);
+    hipLaunchKernelGGL(k_dos_final, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, st, part2, ng, nsp, n_freq,
+                       dos);
+}
+
 // complex128 copy of complex columns [c0, c0 + nc) of float [T][ncs] pairs
 __global__ void k_cols_to_f64(const float2 *__restrict__ in, int64_t T, int64_t ncs, int64_t c0, int64_t nc,
                               double2 *__restrict__ out)
@@ -634,7 +753,10 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
                                         hipStream_t);                                                                 \
     template void launch_iqt<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *, hipStream_t); \
     template void launch_iqt_fft<R>(const SqwParams &, const RadixPlan &, int, const cplx<R>::T *, const double2 *,  \
-                                    const float *, double *, double *, hipStream_t);
+                                    const float *, double *, double *, hipStream_t);                                 \
+    template size_t dos_fused_lds<R>(int64_t, int);                                                                   \
+    template void launch_dos_fused<R>(const DosParams &, const RadixPlan &, int, const float *, const int *,          \
+                                      const cplx<R>::T *, const double *, double *, hipStream_t);
 
 MPSQ_SQW_INST(float)
 MPSQ_SQW_INST(double)
