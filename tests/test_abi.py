@@ -90,3 +90,18 @@ def test_bad_arguments_are_rejected_before_any_device_call(lib):
     assert lib.mpsq_makeplan(1, 3, nm, -1, 0, 1e-6, C.byref(h), None) == _lib.ERR_BAD_ARG  # ntrans 0
     assert lib.mpsq_sqw_create(None, C.byref(h)) == _lib.ERR_BAD_ARG
     assert lib.mpsq_execute(None, None, None) == _lib.ERR_BAD_ARG
+    # the entry points of the rows around the S(Q,w) map
+    dq = (C.c_float * 3)(0.1, 0.1, 0.1)
+    ai = (C.c_float * 9)(1, 0, 0, 0, 1, 0, 0, 0, 1)
+    buf = (C.c_double * 64)()
+    skip = (C.c_int64 * 2)()
+    assert lib.mpsq_sql_hist(None, 1, 1, nm, dq, ai, 1.0, 0.1, 11, buf, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_sql_hist_mc(None, 1, 1, nm, dq, ai, 1.0, 0.1, 11, 1, 10, 1, 0, buf, skip, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_sql_hist_mc(buf, 1, 1, nm, dq, ai, 1.0, 0.1, 11, 0, 10, 1, 0, buf, skip, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_sql_hist_mc(buf, 1, 1, nm, dq, ai, 1.0, 0.1, 11, 1, 10, 1, 0, buf, None, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_dos(None, 8, 4, buf, 1, 5, 0, buf, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_dos(buf, 8, 4, buf, 1, 6, 0, buf, None) == _lib.ERR_BAD_ARG          # n_freq > T / 2 + 1
+    assert lib.mpsq_dos(buf, 8, 4, buf, 1, 5, 7, buf, None) == _lib.ERR_BAD_ARG          # precision
+    assert lib.mpsq_oneph_amplitudes(None, buf, buf, 1, None, None, buf, None) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_speckle_filter(None, 4, 4, 0, 2.0) == _lib.ERR_BAD_ARG
+    assert lib.mpsq_speckle_filter(buf, 0, 4, 0, 2.0) == _lib.ERR_BAD_ARG
