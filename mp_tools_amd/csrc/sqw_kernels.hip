@@ -49,35 +49,32 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     const int T = (int)p.T;
     const int64_t q0 = (int64_t)blockIdx.x * cols;
     const int ncol = (int)min((int64_t)cols, p.NQ - q0);
+    // element i = threadIdx.x + k * NT of the tile is frame i / cols, column
+    // i % cols; NT is a multiple of cols, so a thread's elements all belong
+    // to column c = threadIdx.x % cols (frames seg, seg + nseg, ...)
+    const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
+    const int64_t sc = c < ncol ? src_q(p, q0 + c) : -1;
     {
-        // all loads of a thread in flight before the first LDS store
+        // all loads of a thread in flight before any use; the column sums
+        // from the registers, frames in increasing order, as the values go
+        // to the tile
         C v[E];
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-            const int i = threadIdx.x + k * NT;
-            const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
-            const int64_t s = (i < T * cols && c < ncol) ? src_q(p, q0 + c) : -1;
-            v[k] = (s >= 0) ? amp[(int64_t)t * p.NQs + s] : cmk<C>((R)0, (R)0);
+            const int t = seg + k * nseg;
+            v[k] = (t < T && sc >= 0) ? amp[(int64_t)t * p.NQs + sc] : cmk<C>((R)0, (R)0);
         }
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int i = threadIdx.x + k * NT;
-            if (i < T * cols) tile[i] = v[k];
-        }
-    }
-    __syncthreads();
-    {
-        // thread (c, seg) sums frames seg, seg + nseg, ...; the lanes of a
-        // wave that share c (64 / cols of them) are combined by shuffles,
-        // then the waves' partials in LDS
-        const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
         double sr = 0.0, si = 0.0, ss = 0.0;
-        for (int t = seg; t < T; t += nseg) {
-            const C v = tile[t * cols + c];
-            sr += (double)v.x;
-            si += (double)v.y;
-            ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-        }
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (seg + k * nseg < T) {
+                tile[(seg + k * nseg) * cols + c] = v[k];
+                sr += (double)v[k].x;
+                si += (double)v[k].y;
+                ss += (double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y;
+            }
+        // the lanes of a wave that share c (64 / cols of them) are combined
+        // by shuffles, then the waves' partials in LDS
         for (int o = cols; o < 64; o <<= 1) {
             sr += __shfl_xor(sr, o, 64);
             si += __shfl_xor(si, o, 64);
@@ -104,15 +101,17 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < T * cols; i += NT) {
-        const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
-        const C v = tile[i];
-        double f = win[t];
-        if (qfac && c < ncol) {
-            const int64_t s = src_q(p, q0 + c);
-            if (s >= 0) f *= (double)qfac[s];
+    {
+        // mean subtracted, windowed (and the per-Q factor): each thread its
+        // own elements of the tile
+        const double mr = mean_r[c], mi = mean_i[c];
+        const double qf = (qfac && sc >= 0) ? (double)qfac[sc] : 1.0;
+        for (int t = seg; t < T; t += nseg) {
+            const C x = tile[t * cols + c];
+            double f = win[t];
+            if (qfac && sc >= 0) f *= qf;
+            tile[t * cols + c] = cmk<C>((R)(((double)x.x - mr) * f), (R)(((double)x.y - mi) * f));
         }
-        tile[i] = cmk<C>((R)(((double)v.x - mean_r[c]) * f), (R)(((double)v.y - mean_i[c]) * f));
     }
     __syncthreads();
     lds_fft<C, E>(tile, rp, cols, dcols, cols, 1, tw_l, true);
@@ -130,9 +129,9 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     }
 }
 
-// sliding windows: thread = (Q, group of MG frequencies)
+// sliding windows: thread = (Q, group of MG frequencies; the launch uses 1)
 template <typename R, int MG>
-__global__ __launch_bounds__(256) void k_sqw_sliding(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
+__global__ __launch_bounds__(256, 8) void k_sqw_sliding(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
                                                      const float *__restrict__ qfac, double *__restrict__ S,
                                                      double *__restrict__ Stot)
 {
@@ -422,31 +421,27 @@ __global__ __launch_bounds__(NT) void k_iqt_fft(SqwParams p, RadixPlan rp, int c
     for (int i = threadIdx.x; i < T; i += NT) tw_l[i] = tw[i];
     const int64_t q0 = (int64_t)blockIdx.x * cols;
     const int ncol = (int)min((int64_t)cols, p.NQ - q0);
+    // a thread's elements are column c = threadIdx.x % cols, frames seg,
+    // seg + nseg, ... (NT is a multiple of cols)
+    const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
+    const int64_t sc = c < ncol ? src_q(p, q0 + c) : -1;
     {
         A v[E];
 #pragma unroll
         for (int k = 0; k < E; ++k) {
-            const int i = threadIdx.x + k * NT;
-            const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
-            const int64_t s = (i < T * cols && c < ncol) ? src_q(p, q0 + c) : -1;
-            v[k] = (s >= 0) ? amp[(int64_t)t * p.NQs + s] : cmk<A>((R)0, (R)0);
+            const int t = seg + k * nseg;
+            v[k] = (t < T && sc >= 0) ? amp[(int64_t)t * p.NQs + sc] : cmk<A>((R)0, (R)0);
         }
-#pragma unroll
-        for (int k = 0; k < E; ++k) {
-            const int i = threadIdx.x + k * NT;
-            if (i < T * cols) xt[i] = make_double2((double)v[k].x, (double)v[k].y);
-        }
-    }
-    __syncthreads();
-    {
-        const int c = threadIdx.x % cols, seg = threadIdx.x / cols, nseg = NT / cols;
         double sr = 0.0, si = 0.0, ss = 0.0;
-        for (int t = seg; t < T; t += nseg) {
-            const double2 v = xt[t * cols + c];
-            sr += v.x;
-            si += v.y;
-            ss += v.x * v.x + v.y * v.y;
-        }
+#pragma unroll
+        for (int k = 0; k < E; ++k)
+            if (seg + k * nseg < T) {
+                const double2 x = make_double2((double)v[k].x, (double)v[k].y);
+                xt[(seg + k * nseg) * cols + c] = x;
+                sr += x.x;
+                si += x.y;
+                ss += x.x * x.x + x.y * x.y;
+            }
         for (int o = cols; o < 64; o <<= 1) {
             sr += __shfl_xor(sr, o, 64);
             si += __shfl_xor(si, o, 64);
@@ -474,12 +469,15 @@ __global__ __launch_bounds__(NT) void k_iqt_fft(SqwParams p, RadixPlan rp, int c
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < T * cols; i += NT) {
-        const int t = (int)fdiv((uint32_t)i, dcols), c = i - t * cols;
-        const double2 v = xt[i];
-        const double2 x = make_double2((v.x - mean_r[c]) * qf_s[c], (v.y - mean_i[c]) * qf_s[c]);
-        xt[i] = x;
-        yt[i] = t < p.n_int ? x : make_double2(0.0, 0.0);
+    {
+        // x = (C - mean) qf over all frames, y = x over the first n_int
+        const double mr = mean_r[c], mi = mean_i[c], qf = qf_s[c];
+        for (int t = seg; t < T; t += nseg) {
+            const double2 v = xt[t * cols + c];
+            const double2 x = make_double2((v.x - mr) * qf, (v.y - mi) * qf);
+            xt[t * cols + c] = x;
+            yt[t * cols + c] = t < p.n_int ? x : make_double2(0.0, 0.0);
+        }
     }
     __syncthreads();
     lds_fft<double2, E>(xt, rp, cols, dcols, cols, 1, tw_l, true);
@@ -737,7 +735,12 @@ template <typename R>
 void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S,
                         double *Stot, hipStream_t st)
 {
-    constexpr int MG = 4;
+    // one frequency per thread, at most 64 registers (8 waves per SIMD): the
+    // recurrences of a frequency hold 36 FP64 values, so more frequencies per
+    // thread cost occupancy (a 2.1 M mode map at 17 frequencies, 256
+    // windows: 137.7 ms with 4 per thread, 87.1 with 2, 77.0 with 1 at 116
+    // registers, 50.5 with 1 at 64)
+    constexpr int MG = 1;
     const unsigned gx = (unsigned)((p.NQ + 63) / 64);
     const unsigned gy = (unsigned)((p.n_f + 4 * MG - 1) / (4 * MG));
     hipLaunchKernelGGL((k_sqw_sliding<R, MG>), dim3(gx, gy > 0 ? gy : 1), dim3(256), 0, st, p, amp, qfac, S, Stot);
