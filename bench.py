@@ -324,7 +324,7 @@ def _timed(fn, reps=3):
 
 def next_rows_record(a, dev, eng, amp, pos, counts, nat, dt):
     """Times of the paths either side of the S(Q,w) map on the same workload:
-    an E(Q) cut and I(Q,t) (map at 4 lags by direct sums and at every lag by
+    an E(Q) cut, E = const maps over sliding windows, and I(Q,t) (map at 4 lags by direct sums and at every lag by
     FFT correlation, cut at every lag) from the C2
     amplitudes, the one-phonon amplitudes of the C2 lattice (5 sites x
     40^3 cells, 64 frames) and the mp_dos spectra of its velocities (central
@@ -345,6 +345,11 @@ def next_rows_record(a, dev, eng, amp, pos, counts, nat, dt):
     ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, nat=nat, dt=dt, qidx=qi))
     rec["eq_cut"] = {"pixels": nq, "n_int": n_int, "frequencies": n_fr, "ms": ms,
                      "alg_GBps": (T * nq * 8 + n_fr * nq * 8) / (ms * 1e-3) / 1e9}
+    # E = const maps with the reference's default window (n_int = nfile / 2,
+    # odd: 256 sliding windows), 17 energies around 0 over all modes
+    ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, m_lo=-8, n_f=17, nat=nat, dt=dt), reps=2)
+    rec["econst_map_sliding"] = {"modes": NQ, "n_int": n_int, "windows": T - n_int + 1, "energies": 17, "ms": ms,
+                                 "alg_GBps": (T * NQ * 8 + 18 * NQ * 8) / (ms * 1e-3) / 1e9}
     n_i, n_lag = S.reference_n_int(T, 0, sqw=False)
     ms = _timed(lambda: eng.reduce(amp, n_int=0, odd_n_int=True, kind=2, nat=nat, qidx=qi))
     rec["iqt_cut"] = {"pixels": nq, "n_int": n_i, "lags": n_lag, "ms": ms}
