@@ -1186,7 +1186,7 @@ template <typename R> struct Engine {
         RadixPlan rp{};
         const bool single = (p.n_frame == 1) && (T <= sqw_tile_elems<R>()) &&
                             sqw_single_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
-        if (single) {
+        if (p.n_frame == 1) {
             if (win_n != (int)T || win_kind != r->win_kind) {
                 win_h.resize((size_t)T);
                 for (int64_t i = 0; i < T; ++i) {
@@ -1199,6 +1199,8 @@ template <typename R> struct Engine {
                 win_n = (int)T;
                 win_kind = r->win_kind;
             }
+        }
+        if (single) {
             if (twT_n != (int)T) {
                 upload_twiddles<R>(twT, (int)T, st, twT_h);
                 twT_n = (int)T;
@@ -1210,6 +1212,11 @@ template <typename R> struct Engine {
                 cols *= 2;
             const int ev = tm.begin(ST_REDUCE, st);
             launch_sqw_single<R>(p, rpT, cols, amp, twT.as<Cx<R>>(), win_d.as<double>(), r->d_qfac, d_S, d_Stot, st);
+            tm.end(ev, st);
+        } else if (p.n_frame == 1) {
+            // one window, T without a radix plan: one DFT recurrence per frequency
+            const int ev = tm.begin(ST_REDUCE, st);
+            launch_sqw_window1<R>(p, amp, r->d_qfac, win_d.as<double>(), d_S, d_Stot, st);
             tm.end(ev, st);
         } else {
             const int ev = tm.begin(ST_REDUCE, st);
@@ -2025,9 +2032,9 @@ struct mpsq_sqw_s {
 // all frames, no mean subtraction, Hann window of the (i + 1) / (n + 1) form,
 // norm 1 / T) and the species sums.  With a radix plan for T, one fused
 // kernel per group of blocks (FFT in LDS, per-block species sums) and a
-// fixed-order reduction of the blocks' sums; otherwise the S(Q,w) sliding
-// kernel into |FFT|^2 rows (chunks of columns within 512 MB) and
-// k_dos_species.
+// fixed-order reduction of the blocks' sums; otherwise one DFT recurrence
+// per frequency (k_sqw_window1) into |FFT|^2 rows (chunks of columns within
+// 512 MB) and k_dos_species.
 template <typename R>
 static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t *d_at_ind, int nsp, int64_t n_freq,
                     double *d_dos, hipStream_t st)
@@ -2049,15 +2056,15 @@ static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t 
     const bool single = (T <= sqw_tile_elems<R>()) && dos_fused_lds<R>(T, 1) <= SQW_MAX_LDS && radix_plan((int)T, rp);
     DevBuf tw, win, S, buf;
     int cols = 1;
+    std::vector<double> wh((size_t)T);
+    for (int64_t t = 0; t < T; ++t)
+        wh[t] = (T <= 1) ? 1.0 : 0.5 * (1.0 - std::cos(2.0 * M_PI * (double)(t + 1) / (double)(T + 1)));
+    win.ensure(sizeof(double) * (size_t)T);
+    HIPCHK(hipMemcpyAsync(win.p, wh.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
     if (single) {
         std::vector<Cx<R>> twh;
         upload_twiddles<R>(tw, (int)T, st, twh);
-        std::vector<double> wh((size_t)T);
-        for (int64_t t = 0; t < T; ++t)
-            wh[t] = (T <= 1) ? 1.0 : 0.5 * (1.0 - std::cos(2.0 * M_PI * (double)(t + 1) / (double)(T + 1)));
-        win.ensure(sizeof(double) * (size_t)T);
-        HIPCHK(hipMemcpyAsync(win.p, wh.data(), sizeof(double) * (size_t)T, hipMemcpyHostToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
         while (2 * cols * T <= sqw_tile_elems<R>() && 2 * cols <= SQW_MAX_COLS &&
                dos_fused_lds<R>(T, 2 * cols) <= SQW_MAX_LDS)
             cols *= 2;
@@ -2108,7 +2115,7 @@ static void dos_run(const float *d_vel, int64_t T, int64_t n_tot, const int32_t 
             launch_sqw_single<R>(p, rp, cols, amp, tw.as<Cx<R>>(), win.as<double>(), nullptr, S.as<double>(), nullptr,
                                  st);
         else
-            launch_sqw_sliding<R>(p, amp, nullptr, S.as<double>(), nullptr, st);
+            launch_sqw_window1<R>(p, amp, nullptr, win.as<double>(), S.as<double>(), nullptr, st);
         launch_dos_species(S.as<double>(), nc, c0, n_freq, d_at_ind, nsp, d_dos, st);
         HIPCHK(hipGetLastError());
     }

@@ -272,6 +272,11 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
 template <typename R>
 void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, double *S,
                         double *Stot, hipStream_t st);
+// one window over all frames (n_frame 1) for a T without a radix plan: the
+// window from the table win[n_int] (FP64)
+template <typename R>
+void launch_sqw_window1(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, const double *win,
+                        double *S, double *Stot, hipStream_t st);
 // DOS: dos[f][s] += species sums of (S(f) + S(-f)) / 2 over columns [c0, c0 + ncols)
 void launch_dos_species(const double *S, int64_t ncols, int64_t c0, int64_t n_freq, const int *at_ind, int nsp,
                         double *dos, hipStream_t st);

@@ -129,6 +129,69 @@ __global__ __launch_bounds__(NT) void k_sqw_single(SqwParams p, RadixPlan rp, in
     }
 }
 
+// One window over all T frames (n_frame == 1) when T has no 2,3,5 radix
+// plan: thread = (Q, frequency), X(m) = sum_i w_i C'(i) e^{-2 pi i m i / n}
+// by a running twiddle with the window w from the table (one complex
+// product per frame, against three for the sliding kernel's decomposition of
+// the window into exponentials); |X|^2 and all sums in FP64.
+template <typename R>
+__global__ __launch_bounds__(256, 8) void k_sqw_window1(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
+                                                     const float *__restrict__ qfac, const double *__restrict__ win,
+                                                     double *__restrict__ S, double *__restrict__ Stot)
+{
+    const int64_t q = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    const int mi_ = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (q >= p.NQ || mi_ >= p.n_f) return;
+    const int n = p.n_int;
+    const int64_t qs = src_q(p, q);
+    if (qs < 0) {
+        if (mi_ == 0 && Stot) Stot[q] = qnan();
+        S[(int64_t)mi_ * p.NQ + q] = qnan();
+        return;
+    }
+    amp += qs;
+    double mr = 0.0, mim = 0.0, ss = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const auto v = amp[(int64_t)i * p.NQs];
+        mr += (double)v.x;
+        mim += (double)v.y;
+        ss += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+    if (mi_ == 0 && Stot) Stot[q] = ss * p.norm_tot;
+    if (p.subtract_mean) {
+        mr /= (double)n;
+        mim /= (double)n;
+    } else {
+        mr = mim = 0.0;
+    }
+    const double qf = qfac ? (double)qfac[qs] : 1.0;
+    int64_t m = (p.m_lo + (int64_t)mi_) % n;
+    if (m < 0) m += n;
+    const double om = 6.283185307179586476925286766559 * (double)m / (double)n;
+    const double rr = cos(om), ri = sin(om);
+    double sr = 1.0, si = 0.0, dr = 0.0, di = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const auto v = amp[(int64_t)i * p.NQs];
+        const double f = win[i] * qf;
+        const double cr = ((double)v.x - mr) * f, ci = ((double)v.y - mim) * f;
+        dr += sr * cr - si * ci;
+        di += sr * ci + si * cr;
+        const double nr = sr * rr + si * ri;  // times e^{-i om}
+        si = si * rr - sr * ri;
+        sr = nr;
+    }
+    S[(int64_t)mi_ * p.NQ + q] = (dr * dr + di * di) * p.norm_s;
+}
+
+template <typename R>
+void launch_sqw_window1(const SqwParams &p, const typename cplx<R>::T *amp, const float *qfac, const double *win,
+                        double *S, double *Stot, hipStream_t st)
+{
+    const unsigned gx = (unsigned)((p.NQ + 63) / 64);
+    const unsigned gy = (unsigned)((p.n_f + 3) / 4);
+    hipLaunchKernelGGL((k_sqw_window1<R>), dim3(gx, gy > 0 ? gy : 1), dim3(256), 0, st, p, amp, qfac, win, S, Stot);
+}
+
 // sliding windows: thread = (Q, group of MG frequencies; the launch uses 1)
 template <typename R, int MG>
 __global__ __launch_bounds__(256, 8) void k_sqw_sliding(SqwParams p, const typename cplx<R>::T *__restrict__ amp,
@@ -754,6 +817,8 @@ void launch_sqw_sliding(const SqwParams &p, const typename cplx<R>::T *amp, cons
                                        hipStream_t);                                                                  \
     template void launch_sqw_sliding<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *,    \
                                         hipStream_t);                                                                 \
+    template void launch_sqw_window1<R>(const SqwParams &, const cplx<R>::T *, const float *, const double *,        \
+                                        double *, double *, hipStream_t);                                             \
     template void launch_iqt<R>(const SqwParams &, const cplx<R>::T *, const float *, double *, double *, hipStream_t); \
     template void launch_iqt_fft<R>(const SqwParams &, const RadixPlan &, int, const cplx<R>::T *, const double2 *,  \
                                     const float *, double *, double *, hipStream_t);                                 \

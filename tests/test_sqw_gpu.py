@@ -108,6 +108,8 @@ RED_CASES = [
     (20, 9, 0, 0, 5, True, False),       # sliding windows (reference default shape, odd n_int)
     (21, 10, 1, -3, 7, False, True),
     (12, 1, 0, 0, 1, True, False),       # n_int = 1: window 1
+    (53, 53, 0, 0, 27, True, True),      # one window, T without a 2,3,5 radix plan: DFT recurrence
+    (37, 37, 1, -5, 12, False, False),
     (512, 512, 0, 0, 257, True, False),   # one window over 512 frames (the C2 shape)
     (256, 256, 1, -100, 200, False, True),  # 256 frames, negative m_lo, per-Q factor
     # long single windows: the LDS time FFT at its largest sizes (FP64 2048 / 4096,
