@@ -221,8 +221,20 @@ __global__ void k_sort_count(const int *__restrict__ key, const SortChunk *__res
     const int nbins = ch.nbins;
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) hist[b] = 0;
     __syncthreads();
-    for (int64_t p = ch.p0 + threadIdx.x; p < ch.p1; p += blockDim.x)
-        __hip_atomic_fetch_add(&hist[key[p]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // a wave's lanes hold consecutive points, and points in storage order
+    // (lattice rows) run in the same bin: one atomic per run of equal keys,
+    // adding the run's length (integer counts: the result is the same)
+    const int lane = threadIdx.x & 63;
+    for (int64_t p0 = ch.p0; p0 < ch.p1; p0 += blockDim.x) {
+        const int64_t p = p0 + threadIdx.x;
+        const int k = p < ch.p1 ? key[p] : -1;
+        const int prev = __shfl_up(k, 1, 64);
+        const bool start = lane == 0 || k != prev;
+        const unsigned long long starts = __ballot(start);
+        const unsigned long long above = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+        const int len = (above ? __builtin_ctzll(above) : 64) - lane;
+        if (start && k >= 0) __hip_atomic_fetch_add(&hist[k], len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
     __syncthreads();
     for (int b = threadIdx.x; b < nbins; b += blockDim.x) H[ch.hbase + (int64_t)b * ch.nch + ch.lc] = hist[b];
 }
