@@ -530,7 +530,8 @@ template <typename R> struct Core {
 
     int colfft_cols(int64_t nlen, int64_t inner, bool final) const
     {
-        const int TE = colfft_tile_elems<R>(final);
+        const int TE = (final && nlen <= colfft_final_small_elems<R>()) ? colfft_final_small_elems<R>()
+                                                                         : colfft_tile_elems<R>(final);
         int c = 1;
         while ((int64_t)(2 * c) * nlen <= TE) c *= 2;
         while (c > 1 && c / 2 >= inner) c /= 2;
@@ -808,10 +809,12 @@ template <typename R> struct Plan {
                               run_tab.dev<int>(o_xr), wc.as<int>(), fbl.as<int>(), c.spread_lds_size(), c.st);
         if (!cy.empty())
             launch_colfft_jobs<R>(run_tab.dev<ColJob<R>>(o_cy), run_tab.dev<int>(o_z), 1,
-                                  (int)Core<R>::col_blocks(cy[0]), false, colfft_lds<R>(cy[0].cp, false), c.st);
+                                  (int)Core<R>::col_blocks(cy[0]), false, colfft_lds<R>(cy[0].cp, false), c.st,
+                                  cy[0].cp.n);
         if (!cz.empty())
             launch_colfft_jobs<R>(run_tab.dev<ColJob<R>>(o_cz), run_tab.dev<int>(o_z), 1,
-                                  (int)Core<R>::col_blocks(cz[0]), true, colfft_lds<R>(cz[0].cp, true), c.st);
+                                  (int)Core<R>::col_blocks(cz[0]), true, colfft_lds<R>(cz[0].cp, true), c.st,
+                                  cz[0].cp.n);
         HIPCHK(hipGetLastError());
         if (!on_device) {
             int hbad = 0;
@@ -1480,11 +1483,13 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     tm->end(ev, st);
     if (cyn > 0) {
         ev = tm->begin(ST_COLPASS, st);
-        launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cy), A.dev<int>(o_cyb), (int)cyj.size(), cyn, false, cyl, st);
+        launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cy), A.dev<int>(o_cyb), (int)cyj.size(), cyn, false, cyl, st, 0);
         tm->end(ev, st);
     }
     ev = tm->begin(ST_FINAL, st);
-    launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cz), A.dev<int>(o_czb), (int)czj.size(), czn, true, czl, st);
+    int czmax = 0;
+    for (const ColJob<R> &cj : czj) czmax = std::max(czmax, cj.cp.n);
+    launch_colfft_jobs<R>(A.dev<ColJob<R>>(o_cz), A.dev<int>(o_czb), (int)czj.size(), czn, true, czl, st, czmax);
     tm->end(ev, st);
 }
 

@@ -199,6 +199,9 @@ template <typename R>
 void launch_binmax(const PtRec<R> *srec, const int64_t *binstart, const TransformInfo *tinfo, int ntr, int max_nbins,
                    int *binmax, hipStream_t st);
 template <typename R> int colfft_tile_elems(bool final);
+// columns of length n <= colfft_final_small_elems() take the final pass's
+// smaller tile (and its blocks of fewer threads)
+template <typename R> int colfft_final_small_elems();
 // Spread + x FFT of every tile of a job table in one persistent launch.
 // Workgroups run on XCD blockIdx.x % 8 and take tiles of their XCD's range
 // [xr[x], xr[x+1]) of the tile-id space from the counter
@@ -246,7 +249,7 @@ void launch_absmax(int64_t M, int ntr, const float2 *c, float *amax, hipStream_t
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final);
 template <typename R>
 void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int nblocks, bool final, size_t lds,
-                        hipStream_t st);
+                        hipStream_t st, int max_n);
 
 // --- launch helpers (sqw_kernels.hip) ---
 // X-ray form-factor weights w[s][Q] on the mode grid N (see sqw_kernels.hip)

@@ -569,8 +569,12 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
 template <typename R> struct Tiles;
 // y pass: 1024 threads of EY elements (FP32: 16 columns of 512 rows,
 // 128-byte row segments; two workgroups, 32 waves, per CU); final pass:
-// 512 threads of EF elements, half the tile, so that three workgroups
-// share a CU while each walks its species one after another
+// 256 threads of EF elements when the columns fit (FP32 on C2: 32 columns of
+// 64 rows, 256-byte row segments), small tiles so that many workgroups
+// share a CU while each walks its species one after another (C2, 24
+// launches of 16 frames: 4.63 ms with 256 threads, 5.01 with 512, 4.96 with
+// 128; 4 or 16 elements per thread 5.9-6.2), else 512 threads (the size
+// limit of the column passes)
 // RY: arithmetic of the y pass.  FP32 data keeps FP32 arithmetic: an FP64
 // y pass (8 columns of 512 rows of double2, two workgroups per CU) measured
 // 0.68 against 0.37 ms per 16-frame C2 launch, for errors of the FP32 form
@@ -579,6 +583,8 @@ template <typename R> struct Tiles;
 template <> struct Tiles<float> { static constexpr int EY = 8, EF = 8; using RY = float; };
 template <> struct Tiles<double> { static constexpr int EY = 4, EF = 4; using RY = double; };
 constexpr int colfft_nt(bool final) { return final ? 512 : 1024; }
+// the final pass in 256-thread blocks when every column of the launch fits
+constexpr int COLFFT_NT_FINAL_SMALL = 256;
 
 
 template <typename R>
@@ -675,6 +681,8 @@ template <typename R> int colfft_tile_elems(bool final)
     return (final ? Tiles<R>::EF : Tiles<R>::EY) * colfft_nt(final);
 }
 
+template <typename R> int colfft_final_small_elems() { return Tiles<R>::EF * COLFFT_NT_FINAL_SMALL; }
+
 // LDS of one column pass of a job: tile + twiddles (+ species sum for the final pass)
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
 {
@@ -684,20 +692,28 @@ template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
 
 template <typename R>
 void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int nblocks, bool final, size_t lds,
-                        hipStream_t st)
+                        hipStream_t st, int max_n)
 {
     if (nblocks <= 0) return;
     constexpr int EF = Tiles<R>::EF, EY = Tiles<R>::EY, NTF = colfft_nt(true), NTY = colfft_nt(false);
+    constexpr int NTS = COLFFT_NT_FINAL_SMALL;
     using RY = typename Tiles<R>::RY;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, R, EF, NTF, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, R, EF, NTS, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, RY, EY, NTY, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         attr = true;
     }
-    if (final)
+    // the jobs' column counts were chosen for the small tile when their
+    // columns fit it (colfft_final_small_elems), which the large tile holds too
+    if (final && max_n <= colfft_final_small_elems<R>())
+        hipLaunchKernelGGL((k_colfft<R, R, EF, NTS, true>), dim3((unsigned)nblocks), dim3(NTS), lds, st, jobs, blk0s,
+                           njobs);
+    else if (final)
         hipLaunchKernelGGL((k_colfft<R, R, EF, NTF, true>), dim3((unsigned)nblocks), dim3(NTF), lds, st, jobs, blk0s,
                            njobs);
     else
@@ -717,8 +733,9 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
     template void launch_binmax<R>(const PtRec<R> *, const int64_t *, const TransformInfo *, int, int, int *,         \
                                    hipStream_t);                                                                      \
     template int colfft_tile_elems<R>(bool);                                                                          \
+    template int colfft_final_small_elems<R>();                                                                       \
     template size_t colfft_lds<R>(const ColPass &, bool);                                                             \
-    template void launch_colfft_jobs<R>(const ColJob<R> *, const int *, int, int, bool, size_t, hipStream_t);
+    template void launch_colfft_jobs<R>(const ColJob<R> *, const int *, int, int, bool, size_t, hipStream_t, int);
 
 MPSQ_INST(float)
 MPSQ_INST(double)
