@@ -13,6 +13,17 @@
 //                rectangular sliding DFTs that advance by an O(1) recurrence
 //                per frame (FP64); sum_s |X_s(m)|^2 accumulates in FP64
 //                (mp_sqom56.f90:1437-1478 computes the same sum per energy).
+// k_sqw_window1  one window over all frames when T has no 2,3,5 radix plan:
+//                one DFT recurrence per (Q, frequency), window from a table.
+// k_iqt          I(Q,t) (mp_sqom56.f90:1467-1471, 1504-1510) by direct sums,
+//                a thread per Q and group of lags;
+// k_iqt_fft      the same for many lags by FFT correlation in LDS (FP64).
+// k_dos_fused    mp_dos spectra: time FFT of packed velocity columns and
+//                per-block species sums (mp_dos56.f90:495-525), added in a
+//                fixed order by k_dos_groups / k_dos_final.
+// k_qtables      the Q plane's q2_norm and X-ray form factor tables
+//                (mp_sqom56.f90:901-936); k_xray_table the per-Q X-ray weights
+//                of served requests on the mode grid.
 #include "mpsq_kernels.h"
 
 namespace mpsq {
