@@ -198,3 +198,70 @@ def write_sqom_table(path: str, cs_plot, **header) -> None:
     n_qx, n_qy)`` for the total map or the same of one energy's S(Q, w)."""
     with open(path, "w") as f:
         f.write(sqom_table(cs_plot, **header))
+
+
+def _fixed(x: float, w: int, d: int) -> str:
+    """Fortran Fw.d of a REAL(4) value (asterisks when it does not fit)."""
+    t = "%*.*f" % (w, d, float(np.float32(x)))
+    return t if len(t) <= w else "*" * w
+
+
+def dos_frequencies(n_freq: int, t_step: float) -> np.ndarray:
+    """The frequency column of mp_dos's table: f_max = 0.5 / t_step and
+    ff(j) = f_min + (j - 1) (f_max - f_min) / real(n_freq - 1), f_min = 0, in
+    REAL(4) (mp_dos56.f90:474, 606-607, 620-622) -- the plot axis, which
+    runs to the Nyquist frequency over the n_freq rows."""
+    f32 = np.float32
+    f_max = f32(f32(0.5) / f32(t_step))
+    if n_freq <= 1:
+        return np.zeros(max(n_freq, 0), np.float32)
+    j = np.arange(n_freq, dtype=np.float32)
+    return ((j * f_max).astype(np.float32) / f32(n_freq - 1)).astype(np.float32)
+
+
+def dos_table(cs_out, *, t0: float, t_step: float, n_row, temp: float, at_names, occ, weights,
+              file_dat_t0: str = "", file_dat: str = "", mp_tool: str = "MP_DOS 1.56", shells: bool = False) -> str:
+    """The text of mp_dos's output file (mp_dos56.f90:723-736): header records
+    (tool, input files, t0 / t_step / n_row / n_atom, temperature, atoms,
+    occupations, weights, column names) and one row per frequency,
+    (1x, f6.3, 2x, f9.5, 2x, (1x, f9.5)), of the frequency, the total and the
+    partial DOS.  ``cs_out`` is [n_freq, nsp + 1] as ``sqw.dos_output``
+    returns it (partials, then the total)."""
+    cs = np.asarray(cs_out, np.float64)
+    nsp = len(at_names)
+    if cs.ndim != 2 or cs.shape[1] != nsp + 1:
+        raise ValueError("cs_out must be [n_freq, nsp + 1]")
+    names = [str(n).ljust(4)[:4] for n in at_names]
+    rec = [ld_record("*****    ", mp_tool.strip(), ": total and partial densities of vibrational states     *****"),
+           ld_record()]
+    if shells:
+        rec.append(ld_record("ATTENTION: the DOS corresponds to atom SHELLS!"))
+    rec += [ld_record("Input files:  ", file_dat_t0.strip(), " to ", file_dat.strip()),
+            ld_record("  t0,t_step,n_row,n_atom:", _f32(t0), _f32(t_step), [int(v) for v in n_row], int(nsp)),
+            ld_record("  temperature [K]", _f32(temp)),
+            ld_record(),
+            " Atoms:      " + "   ".join(names),  # (a, 3x): the last 3x writes nothing
+            " Occupations:" + "".join(_fixed(v, 7, 3) for v in occ),
+            " Weights:    " + "".join(_fixed(v, 7, 3) for v in weights),
+            ld_record(),
+            " f [THz] " + "   Total  " + "".join(n.rjust(9) for n in names)]
+    ff = dos_frequencies(cs.shape[0], t_step)
+    for i in range(cs.shape[0]):
+        rec.append(" " + _fixed(ff[i], 6, 3) + "  " + _fixed(cs[i, nsp], 9, 5) + "  "
+                   + "".join(" " + _fixed(cs[i, j], 9, 5) for j in range(nsp)))
+    return "\n".join(rec) + "\n"
+
+
+def dos_file_name(file_master: str, nfile_min: int, nfile: int, jfile: int = 1) -> str:
+    """mp_dos's output name (mp_dos56.f90:683-698): master_dos_<first>_<last>_<jj>.txt,
+    the frame numbers as i4.4 below 10000 (i8, left-adjusted, above)."""
+    def num(v):
+        return "%04d" % v if v <= 9999 else str(int(v))
+    return f"{file_master.strip()}_dos_{num(nfile_min)}_{num(nfile)}_{jfile:02d}.txt"
+
+
+def write_dos_table(path: str, cs_out, **header) -> None:
+    """Write ``dos_table(cs_out, **header)`` to ``path``."""
+    with open(path, "w", encoding="utf-8") as f:
+        f.write(dos_table(cs_out, **header))
+

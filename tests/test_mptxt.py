@@ -136,3 +136,36 @@ def test_qplane_closed_forms():
     g = sqw.qplane((1, 1, 1), 2.0, q_center=(0, 0, 0), a_par=(40.0, 40.0, 40.0), nsuper=1)
     assert g["n_qx"] == int(np.float32(2.0) / np.float32(2 * np.pi / 40.0)) + (int(np.float32(2.0) / np.float32(2 * np.pi / 40.0)) % 2)
     assert g["engine_kw"]["axes"].shape == (3, 3)
+
+
+def test_dos_table_against_the_reference_bzo_file():
+    """mp_dos's table (mp_dos56.f90:723-736) for the reference's BZO example
+    (bzo200d_dos_0001_0020_01.txt, 20 snapshots, t_step 0.04 ps): the header
+    records come out byte for byte, and the row count and frequency field of
+    every row match -- the frequency count int(0.5 (nfile - 1)) + 1 and the
+    REAL(4) axis to 0.5 / t_step (5.555, not 5.556, at t_step = 0.0400009155).
+    The file was written by MP_DOS55, whose value fields had three decimals,
+    so the rows' value fields are not compared."""
+    import json
+    from mp_tools_amd import mptxt, sqw
+    gold = json.load(open(os.path.join(HERE, "golden", "bzo", "bzo_dos_table.json"), encoding="utf-8"))
+    t_step = np.float32(4.00009155e-02)
+    n_freq = sqw.dos_n_freq(20, float(t_step))
+    assert n_freq == gold["n_rows"]
+    cs = np.zeros((n_freq, 6))
+    txt = mptxt.dos_table(cs, t0=64.04, t_step=t_step, n_row=(32, 32, 32), temp=np.float32(199.586273),
+                          at_names=["Ba", "Zr", "O1", "O2", "O3"], occ=[1.0] * 5,
+                          weights=[25.705, 51.266, 33.675, 33.675, 33.675],
+                          file_dat_t0="./data/bzo200d_n0001.dat", file_dat="./data/bzo200d_n0020.dat",
+                          mp_tool="MP_DOS55")
+    lines = txt.split("\n")[:-1]
+    assert lines[:len(gold["header"])] == gold["header"]
+    rows = lines[len(gold["header"]):]
+    assert len(rows) == gold["n_rows"]
+    assert [r[:7] for r in rows] == gold["freq_fields"]
+    assert mptxt.dos_file_name("bzo200d", 1, 20, 1) == "bzo200d_dos_0001_0020_01.txt"
+    # mp_dos56's own row format: (1x, f6.3, 2x, f9.5, 2x, (1x, f9.5))
+    cs[3, :] = [0.1, 0.2, 0.3, 0.4, 0.5, 1.5]
+    r3 = mptxt.dos_table(cs, t0=0, t_step=t_step, n_row=(1, 1, 1), temp=0, at_names=["Ba", "Zr", "O1", "O2", "O3"],
+                         occ=[1] * 5, weights=[1] * 5).split("\n")[11 + 3]
+    assert r3 == " " + " 4.167" + "  " + "  1.50000" + "  " + "".join(" %9.5f" % v for v in (0.1, 0.2, 0.3, 0.4, 0.5))
