@@ -698,16 +698,17 @@ void launch_colfft_jobs(const ColJob<R> *jobs, const int *blk0s, int njobs, int 
     constexpr int EF = Tiles<R>::EF, EY = Tiles<R>::EY, NTF = colfft_nt(true), NTY = colfft_nt(false);
     constexpr int NTS = COLFFT_NT_FINAL_SMALL;
     using RY = typename Tiles<R>::RY;
-    static bool attr = false;
-    if (!attr) {
+    // set once per process (a function-local static initialises thread-safely)
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, R, EF, NTF, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, R, EF, NTS, true>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_colfft<R, RY, EY, NTY, false>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 156 * 1024);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     // the jobs' column counts were chosen for the small tile when their
     // columns fit it (colfft_final_small_elems), which the large tile holds too
     if (final && max_n <= colfft_final_small_elems<R>())

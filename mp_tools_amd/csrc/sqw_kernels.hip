@@ -584,12 +584,13 @@ void launch_iqt_fft(const SqwParams &p, const RadixPlan &rp, int cols, const typ
 {
     constexpr int E = SqwTiles<double>::E, NT = SqwTiles<double>::NT;
     const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
-    static bool attr = false;
-    if (!attr) {
+    // set once per process (a function-local static initialises thread-safely)
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_iqt_fft<R, E, NT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     hipLaunchKernelGGL((k_iqt_fft<R, E, NT>), dim3(nb), dim3(NT), iqt_fft_lds(p.T, cols), st, p, rp, cols,
                        make_fdiv((uint32_t)cols), amp, tw, qfac, S, Stot);
 }
@@ -712,12 +713,13 @@ void launch_dos_fused(const DosParams &p, const RadixPlan &rp, int cols, const f
 {
     constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
     const unsigned nb = (unsigned)((p.nc + cols - 1) / cols);
-    static bool attr = false;
-    if (!attr) {
+    // set once per process (a function-local static initialises thread-safely)
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_dos_fused<R, E, NT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     hipLaunchKernelGGL((k_dos_fused<R, E, NT>), dim3(nb), dim3(NT), dos_fused_lds<R>(p.T, cols), st, p, rp, cols,
                        make_fdiv((uint32_t)cols), reinterpret_cast<const float2 *>(vel),
                        reinterpret_cast<const int4 *>(at_ind), tw, win, part);
@@ -795,12 +797,13 @@ void launch_sqw_single(const SqwParams &p, const RadixPlan &rp, int cols, const 
     constexpr int E = SqwTiles<R>::E, NT = SqwTiles<R>::NT;
     const unsigned nb = (unsigned)((p.NQ + cols - 1) / cols);
     const size_t lds = sqw_single_lds<R>(p.T, cols);  // <= SQW_MAX_LDS (the caller's choice of cols)
-    static bool attr = false;
-    if (!attr) {
+    // set once per process (a function-local static initialises thread-safely)
+    static const bool attr = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_sqw_single<R, E, NT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)SQW_MAX_LDS);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     hipLaunchKernelGGL((k_sqw_single<R, E, NT>), dim3(nb), dim3(NT), lds, st, p, rp, cols, make_fdiv((uint32_t)cols),
                        make_fdiv((uint32_t)p.T), amp, tw, win, qfac, S, Stot);
 }
