@@ -1373,6 +1373,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         int w;
         size_t o_jobs, o_t0, o_xr, lds;
         int njobs;
+        size_t lds_real;  // > 0: the real narrow kernel (FP32, every job's strengths real)
     };
     std::vector<SpreadLaunch> sl;
     int max_tiles = 0;  // tiles of the largest spread launch
@@ -1382,11 +1383,18 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         std::vector<double> cost;
         std::vector<int> ntl;
         int tile = 0;
-        size_t lds = 0;
+        size_t lds = 0, lds_real = 0;
+        bool real = sizeof(R) == 4;
         for (int q = 0; q < np; ++q) {
             const Engine<R> &e = *parts[q].e;
             const Core<R> &c = e.core;
             if (sclass(c) != widths[wi]) continue;
+            // q_center = 0: the strengths (window and weights times
+            // exp(-i 2 pi q0 . r)) are real
+            const int rows = ((c.dim >= 2) ? c.g.by : 1) * ((c.dim >= 3) ? c.g.bz : 1);
+            real = real && e.sp.q0[0] == 0.0 && e.sp.q0[1] == 0.0 && e.sp.q0[2] == 0.0 &&
+                   spread_real_ok(rows, c.g.n1, c.g.w);
+            if (real) lds_real = std::max(lds_real, spread_lds_real(rows, c.g.n1, c.g.w));
             const int ntr = parts[q].b * tpf(q);
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), npts, ws.bad.template as<int>(),
                                          ws.cs.template as<Cx<R>>(), 0,
@@ -1404,7 +1412,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             lds = std::max(lds, c.spread_lds_size());
         }
         sxr[wi] = xcd_ranges(cost, ntl);
-        sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size()});
+        sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size(), real ? lds_real : 0});
         max_tiles = std::max(max_tiles, tile);
     }
     ws.fbl.ensure(sizeof(int) * (size_t)spread_max_wide_passes() * (size_t)std::max(max_tiles, 1));
@@ -1479,7 +1487,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     ev = tm->begin(ST_SPREAD, st);
     for (const SpreadLaunch &L : sl)
         launch_spread_jobs<R>(A.dev<SpreadJob<R>>(L.o_jobs), A.dev<int>(L.o_t0), L.njobs, L.w, A.dev<int>(L.o_xr),
-                              ws.wc.template as<int>(), ws.fbl.template as<int>(), L.lds, st);
+                              ws.wc.template as<int>(), ws.fbl.template as<int>(), L.lds, st, L.lds_real);
     tm->end(ev, st);
     if (cyn > 0) {
         ev = tm->begin(ST_COLPASS, st);

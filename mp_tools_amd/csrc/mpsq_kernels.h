@@ -192,6 +192,11 @@ template <typename R> int spread_group_rows(int rows, int n1);
 template <typename R> size_t spread_lds(int rows, int n1, int w);
 // byte offset of the twiddles in a spread workgroup's dynamic LDS
 template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w);
+// FP32 tiles of real strengths: whether a geometry can take the real narrow
+// kernel (rows in pairs, one 32-bit cell per row cell, two rows per complex
+// x FFT) and that kernel's dynamic LDS
+bool spread_real_ok(int rows, int n1, int w);
+size_t spread_lds_real(int rows, int n1, int w);
 // binmax[bs0 + b] = largest number of bin b's points whose first kernel cell
 // lies in 3 cyclically consecutive x segments (a bound on how many of them
 // cover any one cell of a row)
@@ -209,10 +214,14 @@ template <typename R> int colfft_final_small_elems();
 // tiles too dense for 32-bit accumulation are queued in fbl
 // (spread_max_wide_passes() ints per tile of the table) and run by a second,
 // 64-bit launch.  All jobs share the
-// kernel width w; lds = max over jobs of spread_lds.
+// kernel width w; lds = max over jobs of spread_lds.  lds_real > 0 (FP32
+// only): every job's strengths are real (imaginary parts exactly zero; a
+// point with a nonzero one raises err) and every job's geometry passes
+// spread_real_ok, and the narrow launch takes the real form with lds_real =
+// max over jobs of spread_lds_real.
 template <typename R>
 void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, int w, const int *xr, int *wc,
-                        int *fbl, size_t lds, hipStream_t st);
+                        int *fbl, size_t lds, hipStream_t st, size_t lds_real = 0);
 // |Q| histogram of partial intensities (mp_sql's Gauss integration)
 struct SqlGeom {
     int N[3];
