@@ -405,8 +405,12 @@ template <typename R> struct Core {
     bool own_stream = false;
     int warn = 0;
     int sort_chunk = 0;
+    // FP32 with real strengths: the narrow spread kernel's real form, whose
+    // 32-bit cells and paired-row x FFT allow tiles of twice the rows
+    bool real = false;
+    int rows_max = 1;
 
-    void setup(int dim_, const int64_t *nm, int iflag_, double eps_, const mpsq_opts *o)
+    void setup(int dim_, const int64_t *nm, int iflag_, double eps_, const mpsq_opts *o, bool real_ = false)
     {
         if (dim_ < 1 || dim_ > 3 || !nm) throw Err{MPSQ_ERR_BAD_ARG};
         dim = dim_;
@@ -454,7 +458,8 @@ template <typename R> struct Core {
         // spread tile: whole x extent x (by x bz) rows within the LDS tile;
         // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
         // one tile contiguous
-        const int rows_max = spread_max_rows<R>((int)n[0]);
+        real = real_ && sizeof(R) == 4;
+        rows_max = spread_max_rows<R>((int)n[0]) * (real ? 2 : 1);
         const int bmax = spread_max_tile_b();
         int by = 1, bz = 1;
         if (dim == 2) {
@@ -472,7 +477,7 @@ template <typename R> struct Core {
                 }
         }
         if (o->tile_by <= 0 && o->tile_bz <= 0) {
-            const TuneKey key{{dim, (int)n[0], (int)n[1], (int)n[2], w, sizeof(R) == 8 ? 1 : 0}};
+            const TuneKey key{{dim, (int)n[0], (int)n[1], (int)n[2], w, sizeof(R) == 8 ? 1 : (real ? 2 : 0)}};
             std::lock_guard<std::mutex> lk(g_tune_mu);
             auto it = g_tuned.find(key);
             if (it != g_tuned.end()) {
@@ -990,7 +995,10 @@ template <typename R> struct Engine {
     {
         if (!s || (s->dim != 2 && s->dim != 3) || s->nsp < 1 || s->nsp > MAXSP || !s->sp_count || s->n_tot < 1)
             throw Err{MPSQ_ERR_BAD_ARG};
-        core.setup(s->dim, s->n_modes, s->iflag, s->eps, &s->opts);
+        // q_center = 0: every strength (window and weights times
+        // exp(-i 2 pi q0 . r)) is real
+        core.setup(s->dim, s->n_modes, s->iflag, s->eps, &s->opts,
+                   s->q0[0] == 0.0 && s->q0[1] == 0.0 && s->q0[2] == 0.0);
         precision = s->precision;
         n_tot = s->n_tot;
         nsp = s->nsp;
@@ -1361,9 +1369,11 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     // and by how many workgroups of their LDS size fit a CU (a launch's LDS
     // is the largest of its jobs, so a small-grid job launched with a large
     // one would run at the large one's occupancy); one launch per group.
+    // (real and complex strengths never share a launch: a real geometry may
+    // have more rows than the complex kernel takes)
     auto sclass = [&](const Core<R> &c) {
         const int res = (int)std::min<size_t>(8, (size_t)(156 * 1024) / std::max<size_t>(c.spread_lds_size(), 1));
-        return c.g.w * 16 + res;
+        return (c.real ? 4096 : 0) + c.g.w * 16 + res;
     };
     std::vector<int> widths;  // group keys
     for (int q = 0; q < np; ++q)
@@ -1389,11 +1399,8 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             const Engine<R> &e = *parts[q].e;
             const Core<R> &c = e.core;
             if (sclass(c) != widths[wi]) continue;
-            // q_center = 0: the strengths (window and weights times
-            // exp(-i 2 pi q0 . r)) are real
             const int rows = ((c.dim >= 2) ? c.g.by : 1) * ((c.dim >= 3) ? c.g.bz : 1);
-            real = real && e.sp.q0[0] == 0.0 && e.sp.q0[1] == 0.0 && e.sp.q0[2] == 0.0 &&
-                   spread_real_ok(rows, c.g.n1, c.g.w);
+            real = real && c.real && spread_real_ok(rows, c.g.n1, c.g.w);
             if (real) lds_real = std::max(lds_real, spread_lds_real(rows, c.g.n1, c.g.w));
             const int ntr = parts[q].b * tpf(q);
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), npts, ws.bad.template as<int>(),
@@ -1412,7 +1419,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             lds = std::max(lds, c.spread_lds_size());
         }
         sxr[wi] = xcd_ranges(cost, ntl);
-        sl.push_back(SpreadLaunch{widths[wi] / 16, 0, 0, 0, lds, (int)sjobs[wi].size(), real ? lds_real : 0});
+        sl.push_back(SpreadLaunch{(widths[wi] % 4096) / 16, 0, 0, 0, lds, (int)sjobs[wi].size(), real ? lds_real : 0});
         max_tiles = std::max(max_tiles, tile);
     }
     ws.fbl.ensure(sizeof(int) * (size_t)spread_max_wide_passes() * (size_t)std::max(max_tiles, 1));
@@ -1700,7 +1707,7 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     int bb[2] = {c.g.by, c.g.bz};
     for (int by = 1; by <= (dim >= 2 ? bmax : 1); by *= 2)
         for (int bz = 1; bz <= (dim >= 3 ? bmax : 1); bz *= 2) {
-            if (by * bz > spread_max_rows<R>(n1)) continue;
+            if (by * bz > c.rows_max) continue;
             if (dim >= 2 && (n2 % by || by + w > n2)) continue;
             if (dim >= 3 && (n3 % bz || bz + w > n3)) continue;
             if (spread_lds<R>(by * bz, n1, w) > 156 * 1024) continue;
@@ -1725,7 +1732,7 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     {
-        const TuneKey key{{dim, n1, n2, n3, w, sizeof(R) == 8 ? 1 : 0}};
+        const TuneKey key{{dim, n1, n2, n3, w, sizeof(R) == 8 ? 1 : (c.real ? 2 : 0)}};
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[key] = std::make_pair(bb[0], bb[1]);
     }

@@ -72,7 +72,7 @@ template <> struct SpreadCfg<double> { static constexpr int E = 8, NT = 512; };
 struct NarrowCfg {
     static constexpr int E = 8, NT = 512;
 };
-constexpr int MAX_WIDE_PASSES = 4;  // narrow tile rows / wide pass rows
+constexpr int MAX_WIDE_PASSES = 8;  // narrow tile rows / wide pass rows (32-row real tiles: 8)
 
 constexpr int MAX_TILE_B = 16;   // by, bz <= 16
 constexpr int NARROW_MAX = 256;  // largest covering bound of a 32-bit (narrow) FP32 tile

@@ -75,3 +75,24 @@ def test_real_form_equals_complex_form(gpu_lib, nm, iflag):
     a, b = real.cpu().numpy(), cplx.cpu().numpy()
     assert np.all(np.isfinite(a))
     assert rel_l2(a, b) < 1e-6
+
+
+def test_real_dense_tiles_take_the_wide_kernel(gpu_lib):
+    """Points dense enough that tiles exceed the narrow kernel's covering
+    bound: those tiles (32 rows with real strengths) are queued for the
+    64-bit wide kernel in 8 passes of 4 rows; the map still matches the
+    oracle."""
+    import torch
+    from mp_tools_amd.sqw import SqwEngine
+    rng = np.random.default_rng(3)
+    ncell, T, n = 4, 2, 12000
+    counts = np.array([n // 2, n // 2])
+    pos = np.zeros((T, n, 4), np.float32)
+    pos[:, :, :3] = rng.uniform(0.0, ncell, (T, n, 3))
+    nm = (64, 40, 24)
+    w = np.array([0.6, -0.4])
+    eng = SqwEngine(3, nm, counts, eps=1e-6, precision="fp32", weights=w, tpe=(2 * np.pi / ncell,) * 3)
+    amp = eng.amplitudes(torch.tensor(pos, device="cuda"))
+    torch.cuda.synchronize()
+    F = _oracle(pos, counts, nm, ncell, False)
+    assert rel_l2(amp.cpu().numpy(), (F * w[None, :, None]).sum(axis=1)) < 2e-5
