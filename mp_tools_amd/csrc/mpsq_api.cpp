@@ -545,23 +545,38 @@ template <typename R> struct Core {
 
     // Column passes after spreading.  2D: one final pass along y.  3D: a
     // y pass (every transform) and a final z pass that groups nsp
-    // consecutive transforms (species) with their weights.
-    ColPass colpass(int axis, bool final, int nsp) const
+    // consecutive transforms (species) with their weights.  herm (real
+    // strengths): the half spectrum, x frequencies k1 <= 0 (h1x of them)
+    // and y, z frequencies -N/2 .. N - N/2 (one more than kept, so that
+    // every mode's mirror is among them); the final pass writes the mirrors.
+    int64_t h1x(bool herm) const { return herm ? N[0] / 2 + 1 : N[0]; }
+    ColPass colpass(int axis, bool final, int nsp, bool herm = false) const
     {
         ColPass c{};
         const int64_t NQ = N[0] * N[1] * N[2];
+        const int64_t H1 = h1x(herm), N2o = herm ? N[1] + 1 : N[1];
         c.n = (int)n[axis];
         c.N = (int)N[axis];
+        c.NO = herm ? c.N + 1 : c.N;
         if (axis == 1) {
-            c.inner = (int)N[0];
+            c.inner = (int)H1;
             c.outer = (dim == 3) ? (int)n[2] : 1;
-            c.in_tstride = n[2] * n[1] * N[0];
-            c.out_tstride = (dim == 3) ? n[2] * N[1] * N[0] : N[1] * N[0];
+            c.in_tstride = n[2] * n[1] * H1;
+            c.out_tstride = (dim == 3) ? n[2] * N2o * H1 : NQ;
         } else {
-            c.inner = (int)(N[0] * N[1]);
+            c.inner = (int)(N2o * H1);
             c.outer = 1;
-            c.in_tstride = n[2] * N[1] * N[0];
+            c.in_tstride = n[2] * N2o * H1;
             c.out_tstride = NQ;
+        }
+        if (herm && final) {
+            c.herm = 1;
+            c.hmid = (dim == 3) ? 1 : 0;
+            c.hN1 = (int)N[0];
+            c.hH1 = (int)H1;
+            c.hN2 = (dim == 3) ? (int)N[1] : 1;
+            c.hsa = (dim == 3) ? N[0] * N[1] : N[0];
+            c.dH1 = make_fdiv((uint32_t)H1);
         }
         c.cols = colfft_cols(c.n, c.inner, final);
         c.dcols = make_fdiv((uint32_t)c.cols);
@@ -571,10 +586,10 @@ template <typename R> struct Core {
         return c;
     }
     ColJob<R> coljob(int axis, bool final, int nsp, int ngroups, const Cx<R> *in, Cx<R> *out, const R *wsc,
-                     const float *wtab) const
+                     const float *wtab, bool herm = false) const
     {
         ColJob<R> J{};
-        J.cp = colpass(axis, final, nsp);
+        J.cp = colpass(axis, final, nsp, herm);
         J.rp = rp[axis];
         J.in = in;
         J.out = out;
@@ -640,6 +655,7 @@ template <typename R> struct Core {
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
         J.lds_tw = (int)spread_lds_twiddles<R>(by * bz, g.n1, g.w);
+        J.g1_len = g.N1;
         return J;
     }
     // estimated cost of one tile of this geometry with M points per transform
@@ -1279,6 +1295,16 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         const Engine<R> &e = *parts[q].e;
         merged[q] = e.combine && !part_wtab(q);
     }
+    // Real strengths: the real form of the narrow spread (rows in pairs),
+    // and, without a per-Q weight table, the half spectrum through the
+    // column passes (F(-k) = conj F(k); the final pass writes the mirrors)
+    std::vector<char> real_ok(np), herm(np);
+    for (int q = 0; q < np; ++q) {
+        const Core<R> &c = parts[q].e->core;
+        const int rows = ((c.dim >= 2) ? c.g.by : 1) * ((c.dim >= 3) ? c.g.bz : 1);
+        real_ok[q] = c.real && spread_real_ok(rows, c.g.n1, c.g.w);
+        herm[q] = real_ok[q] && !part_wtab(q);
+    }
     auto tpf = [&](int q) { return merged[q] ? 1 : parts[q].e->nact; };  // transforms per frame
     for (int q = 0; q < np; ++q) {
         const Engine<R> &e = *parts[q].e;
@@ -1286,8 +1312,10 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         const int ntr = parts[q].b * tpf(q);
         p0[q + 1] = p0[q] + (int64_t)parts[q].b * e.Mact;
         t0[q + 1] = t0[q] + ntr;
-        g1o[q + 1] = g1o[q] + (int64_t)ntr * c.n[2] * c.n[1] * c.N[0];
-        g2o[q + 1] = g2o[q] + (c.dim == 3 ? (int64_t)ntr * c.n[2] * c.N[1] * c.N[0] : 0);
+        g1o[q + 1] = g1o[q] + (int64_t)ntr * c.n[2] * c.n[1] * c.h1x(herm[q]);
+        g2o[q + 1] = g2o[q] + (c.dim == 3 ? (int64_t)ntr * c.n[2] * (herm[q] ? (c.N[1] + 1) * c.h1x(true)
+                                                                           : c.N[1] * c.N[0])
+                                          : 0);
         bso[q + 1] = bso[q] + (int64_t)ntr * (c.g.nbins + 1);
         max_nbins = std::max(max_nbins, c.g.nbins);
         StageJob &J = stj[q];
@@ -1371,14 +1399,14 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
     // one would run at the large one's occupancy); one launch per group.
     // (real and complex strengths never share a launch: a real geometry may
     // have more rows than the complex kernel takes)
-    auto sclass = [&](const Core<R> &c) {
+    auto sclass = [&](int q) {
+        const Core<R> &c = parts[q].e->core;
         const int res = (int)std::min<size_t>(8, (size_t)(156 * 1024) / std::max<size_t>(c.spread_lds_size(), 1));
-        return (c.real ? 4096 : 0) + c.g.w * 16 + res;
+        return (real_ok[q] ? 4096 : 0) + c.g.w * 16 + res;
     };
     std::vector<int> widths;  // group keys
     for (int q = 0; q < np; ++q)
-        if (std::find(widths.begin(), widths.end(), sclass(parts[q].e->core)) == widths.end())
-            widths.push_back(sclass(parts[q].e->core));
+        if (std::find(widths.begin(), widths.end(), sclass(q)) == widths.end()) widths.push_back(sclass(q));
     struct SpreadLaunch {
         int w;
         size_t o_jobs, o_t0, o_xr, lds;
@@ -1398,9 +1426,9 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         for (int q = 0; q < np; ++q) {
             const Engine<R> &e = *parts[q].e;
             const Core<R> &c = e.core;
-            if (sclass(c) != widths[wi]) continue;
+            if (sclass(q) != widths[wi]) continue;
             const int rows = ((c.dim >= 2) ? c.g.by : 1) * ((c.dim >= 3) ? c.g.bz : 1);
-            real = real && c.real && spread_real_ok(rows, c.g.n1, c.g.w);
+            real = real && real_ok[q];
             if (real) lds_real = std::max(lds_real, spread_lds_real(rows, c.g.n1, c.g.w));
             const int ntr = parts[q].b * tpf(q);
             SpreadJob<R> J = c.spreadjob(ntr, ws.srec.template as<PtRec<R>>(), npts, ws.bad.template as<int>(),
@@ -1410,6 +1438,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
                                          ws.G1.template as<Cx<R>>() + g1o[q], nullptr, 0);
             J.amax_s = wbound[q];
             J.tmajor = merged[q] ? 1 : 0;
+            J.g1_len = (int)c.h1x(herm[q]);
             J.tile0 = tile;
             tile += J.ntiles * ntr;
             sjobs[wi].push_back(J);
@@ -1441,20 +1470,20 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         const R *wsc = (e.combine && !merged[q]) ? (parts[q].wsc ? parts[q].wsc : e.wsc.template as<R>()) : nullptr;
         const float *wtab = part_wtab(q);
         if (c.dim == 3) {
-            ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr);
+            ColJob<R> y = c.coljob(1, false, 1, ntr, g1, g2, nullptr, nullptr, herm[q]);
             y.blk0 = cyn;
             cyn += (int)Core<R>::col_blocks(y);
             cyl = std::max(cyl, colfft_lds<R>(y.cp, false));
             cyj.push_back(y);
             cyb.push_back(y.blk0);
-            ColJob<R> z = c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab);
+            ColJob<R> z = c.coljob(2, true, nspg, ngrp, g2, out, wsc, wtab, herm[q]);
             z.blk0 = czn;
             czn += (int)Core<R>::col_blocks(z);
             czl = std::max(czl, colfft_lds<R>(z.cp, true));
             czj.push_back(z);
             czb.push_back(z.blk0);
         } else {
-            ColJob<R> z = c.coljob(1, true, nspg, ngrp, g1, out, wsc, wtab);
+            ColJob<R> z = c.coljob(1, true, nspg, ngrp, g1, out, wsc, wtab, herm[q]);
             z.blk0 = czn;
             czn += (int)Core<R>::col_blocks(z);
             czl = std::max(czl, colfft_lds<R>(z.cp, true));

@@ -71,6 +71,18 @@ struct ColPass {
     int nsp;
     int64_t NQ;
     int conj;
+    // Outputs kept per column: N, or N + 1 for a half-spectrum (Hermitian)
+    // pass, which also keeps k = N - N/2 so that every mode's mirror -k is
+    // among the kept ones.
+    int NO;
+    // herm (final pass): the columns are the half spectrum of a transform of
+    // real strengths, x frequencies kh in [0, hH1) (k1 = kh - N1/2 <= 0) and,
+    // with hmid, middle (y) indices in [0, hN2 + 1); the pass writes each
+    // kept mode and its mirror F(-k) = conj F(k) into the full [N][hN2][hN1]
+    // map (hsa: the map stride of the transformed axis)
+    int herm, hmid, hN1, hH1, hN2;
+    int64_t hsa;
+    FDiv dH1;
 };
 
 // ---------------------------------------------------------------------------
@@ -117,7 +129,7 @@ template <typename R> struct SpreadJob {
     int bin_tstride;
     const typename cplx<R>::T *tw1;
     const R *phinv1;
-    typename cplx<R>::T *G1;             // [ntr][n3][n2][N1]
+    typename cplx<R>::T *G1;             // [ntr][n3][n2][g1_len]
     const float *amax;                   // per transform, or nullptr (strengths bounded by amax_s)
     float amax_s;
     const int *binmax;                   // FP32: per-bin covering bound, indexed like binstart (or nullptr)
@@ -127,6 +139,7 @@ template <typename R> struct SpreadJob {
     int tmajor;                          // tile ids transform-major (else position-major)
     FDiv dntr, dnty, dby, dnt;           // ntr, tiles along y, by, ntiles
     int lds_tw;                          // byte offset of the twiddles in dynamic LDS
+    int g1_len;                          // G1 row length: N1, or N1 / 2 + 1 (half spectrum, real strengths)
 };
 
 // one column-FFT pass of one job: blocks [blk0, blk0 + ncb*outer*ngroups)

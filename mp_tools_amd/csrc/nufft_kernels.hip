@@ -464,9 +464,10 @@ __global__ void k_absmax(int64_t M, const float2 *__restrict__ c, unsigned *__re
 // ---------------------------------------------------------------------------
 // column FFT pass (y or z): transform t holds [outer][n][inner]; the block
 // transforms `cols` adjacent columns of one outer slab along n in LDS,
-// keeps N of the n frequencies, deconvolves.  FINAL: additionally sums the
-// nsp transforms of group blockIdx.y with weights and writes [g][N][inner]
-// (conjugated for iflag > 0).
+// keeps NO (N, or N + 1 in a half-spectrum pass) of the n frequencies,
+// deconvolves.  FINAL: additionally sums the nsp transforms of group
+// blockIdx.y with weights and writes [g][N][inner] (conjugated for
+// iflag > 0), or, herm, each kept mode and its mirror into the full map.
 // ---------------------------------------------------------------------------
 // RC is the arithmetic (and LDS tile) precision; the data in HBM stays
 // cplx<R> (RC = double on FP32 data runs the pass in FP64 with FP64
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
     C *tile = reinterpret_cast<C *>(smem_raw);
     const int n = cp.n, N = cp.N, cols = cp.cols, inner = cp.inner;
     C *tw_l = tile + n * cols;  // twiddles in LDS
-    C *acc = tw_l + n;          // FINAL: species sum of the kept modes, [N][cols]
+    C *acc = tw_l + n;          // FINAL: species sum of the kept modes, [NO][cols]
     const C *twg;
     if constexpr (sizeof(RC) == sizeof(R)) twg = reinterpret_cast<const C *>(J.tw);
     else twg = reinterpret_cast<const C *>(J.tw64);
@@ -495,8 +496,12 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
     const int o = rem / ncb, cb = rem - o * ncb;
     const int col0 = cb * cols;
     const int ncol = min(cols, inner - col0);
-    const int h = N / 2;
+    const int h = N / 2, NO = cp.NO;
     const int nsp = FINAL ? cp.nsp : 1;
+    // deconvolution factor of kept output kk: phinv is even in k, so the
+    // extra half-spectrum output k = N - h takes that of -k (entry 0 for
+    // even N; for odd N that output is never a mirror source)
+    auto phi_ix = [&](int kk) { return kk < N ? kk : 0; };
     for (int s = 0; s < nsp; ++s) {
         const int t = FINAL ? grp * cp.nsp + s : grp;
         {
@@ -528,21 +533,22 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
         __syncthreads();
         lds_fft<C, E>(tile, J.rp, cols, cp.dcols, cols, 1, tw_l, true);
         if constexpr (!FINAL) {
-            CS *dst = J.out + (int64_t)t * cp.out_tstride + (int64_t)o * N * inner + col0;
-            for (int i = threadIdx.x; i < N * cols; i += NT) {
+            CS *dst = J.out + (int64_t)t * cp.out_tstride + (int64_t)o * NO * inner + col0;
+            for (int i = threadIdx.x; i < NO * cols; i += NT) {
                 const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 if (c >= ncol) continue;
                 int f = kk - h;
                 if (f < 0) f += n;
-                const C x = cscale(tile[f * cols + c], (RC)J.phinv[kk]);
+                const C x = cscale(tile[f * cols + c], (RC)J.phinv[phi_ix(kk)]);
                 dst[(int64_t)kk * inner + c] = cmk<CS>((R)x.x, (R)x.y);
             }
         } else {
-            for (int i = threadIdx.x; i < N * cols; i += NT) {
+            for (int i = threadIdx.x; i < NO * cols; i += NT) {
                 const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
                 int f = kk - h;
                 if (f < 0) f += n;
-                R wv = J.phinv[kk];
+                R wv = J.phinv[phi_ix(kk)];
+                // (a half-spectrum pass has no per-Q table: the host keeps those jobs full)
                 if (J.wtab) wv *= (c < ncol) ? (R)J.wtab[(int64_t)s * cp.NQ + (int64_t)kk * inner + col0 + c] : (R)0;
                 else if (J.wsc) wv *= J.wsc[s];
                 const C x = cscale(tile[f * cols + c], wv);
@@ -552,13 +558,38 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
         __syncthreads();
     }
     if constexpr (FINAL) {
-        C *dst = J.out + (int64_t)grp * cp.out_tstride + (int64_t)o * N * inner + col0;
-        for (int i = threadIdx.x; i < N * cols; i += NT) {
-            const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
-            if (c >= ncol) continue;
-            C x = acc[i];
-            if (cp.conj) x.y = -x.y;
-            dst[(int64_t)kk * inner + c] = x;
+        if (!cp.herm) {
+            C *dst = J.out + (int64_t)grp * cp.out_tstride + (int64_t)o * N * inner + col0;
+            for (int i = threadIdx.x; i < N * cols; i += NT) {
+                const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
+                if (c >= ncol) continue;
+                C x = acc[i];
+                if (cp.conj) x.y = -x.y;
+                dst[(int64_t)kk * inner + c] = x;
+            }
+        } else {
+            // column col = km * hH1 + kh (3D; 2D: col = kh); the transformed
+            // axis' index kk.  Mode (kk, km, kh) goes to its place and, when
+            // k1 < 0, its mirror (2 h - kk, 2 h2 - km, 2 h1 - kh), where in
+            // range, as the conjugate: every mode of the map is written once.
+            C *dst = J.out + (int64_t)grp * cp.out_tstride;
+            const int N1 = cp.hN1, h1 = N1 / 2, N2 = cp.hN2, h2 = N2 / 2;
+            for (int i = threadIdx.x; i < NO * cols; i += NT) {
+                const int kk = (int)fdiv((uint32_t)i, cp.dcols), c = i - kk * cols;
+                if (c >= ncol) continue;
+                const int col = col0 + c;
+                int km = 0, kh = col;
+                if (cp.hmid) {
+                    km = (int)fdiv((uint32_t)col, cp.dH1);
+                    kh = col - km * cp.hH1;
+                }
+                C x = acc[i];
+                if (cp.conj) x.y = -x.y;
+                if (kk < N && km < N2) dst[(int64_t)kk * cp.hsa + (int64_t)km * N1 + kh] = x;
+                const int ka = 2 * h - kk, kmm = cp.hmid ? 2 * h2 - km : 0, kx = 2 * h1 - kh;
+                if (kh < h1 && kx < N1 && ka >= 0 && ka < N && kmm >= 0 && kmm < N2)
+                    dst[(int64_t)ka * cp.hsa + (int64_t)kmm * N1 + kx] = cmk<C>(x.x, -x.y);
+            }
         }
     }
 }
@@ -687,7 +718,7 @@ template <typename R> int colfft_final_small_elems() { return Tiles<R>::EF * COL
 template <typename R> size_t colfft_lds(const ColPass &cp, bool final)
 {
     const size_t es = final ? sizeof(typename cplx<R>::T) : sizeof(typename cplx<typename Tiles<R>::RY>::T);
-    return ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.N * cp.cols : 0)) * es;
+    return ((size_t)cp.n * (cp.cols + 1) + (final ? (size_t)cp.NO * cp.cols : 0)) * es;
 }
 
 template <typename R>

@@ -530,11 +530,12 @@ __device__ __forceinline__ void store_kept(const SpreadJob<R> &J, int t, int y0,
     if (f < N1 - h1) kk = f + h1;
     else if (f >= n1 - h1) kk = f - (n1 - h1);
     else return;
+    if (kk >= J.g1_len) return;  // half spectrum: k1 <= 0 only
     v = cscale(v, J.phinv1[kk]);
     if (e2) v = cmk<typename cplx<R>::T>(ldexp(v.x, e2), ldexp(v.y, e2));
     if (J.final_conj) v.y = -v.y;
     const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
-    J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1 + kk] = v;
+    J.G1[(((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * J.g1_len + kk] = v;
 }
 
 template <typename R, int NT>
@@ -669,7 +670,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
             for (int i = threadIdx.x; i < N1; i += NT) phs[i] = ldexpf(gload(J.phinv1 + i), -q - (REAL ? 1 : 0));
             for (int r = threadIdx.x; r < rows; r += NT) {
                 const int rz = (int)fdiv((uint32_t)r, J.dby), ry = r - rz * by;
-                rowoff[r] = (((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * N1;
+                rowoff[r] = (((int64_t)t * g.n3 + z0 + rz) * g.n2 + y0 + ry) * J.g1_len;
             }
         } else if (threadIdx.x == 0) {
             const int np = rows >> J.grp_sh;  // wide passes of G rows
@@ -805,6 +806,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 auto store_tile = [&](int row, int i, C v) { tile[__mul24(row, RSX) + padx<true>(i)] = v; };
                 constexpr int NW = NT / 64;
                 const int pairs = rows >> 1;
+                const int KO = J.g1_len;  // kept outputs per row (N1, or the half spectrum k1 <= 0)
                 const bool cj = J.final_conj != 0;
                 C *G1 = J.G1;
                 // kept frequency kk of rows 2 pr and 2 pr + 1 from Z = FFT(row 2 pr + i row 2 pr + 1)
@@ -823,13 +825,13 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                     x_fft_w<C, 2 * E, true>(J.rp1, n1, pairs, wv, NW, tw, load_pair, load_tile, store_tile, store_tile);
                     wave_lds_sync();
                     for (int pr = wv; pr < pairs; pr += NW)
-                        for (int kk = lane; kk < N1; kk += 64) split(pr, kk);
+                        for (int kk = lane; kk < KO; kk += 64) split(pr, kk);
                 } else {
                     x_fft<C, 2 * E, NT, true>(J.rp1, n1, pairs, tw, load_pair, load_tile, store_tile, store_tile);
                     __syncthreads();
-                    for (int i = threadIdx.x; i < pairs * N1; i += NT) {
-                        const int pr = i / N1;
-                        split(pr, i - pr * N1);
+                    for (int i = threadIdx.x; i < pairs * KO; i += NT) {
+                        const int pr = i / KO;
+                        split(pr, i - pr * KO);
                     }
                 }
             }
@@ -856,6 +858,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 if (f < N1 - h1) kk = f + h1;
                 else if (f >= n1 - h1) kk = f - (n1 - h1);
                 else return;
+                if (kk >= J.g1_len) return;
                 const float sc = phs[kk];
                 gstore(G1 + rowoff[row] + kk, cmk<C>(v.x * sc, cj ? -(v.y * sc) : v.y * sc));
             };
