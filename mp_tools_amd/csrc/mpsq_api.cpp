@@ -549,7 +549,10 @@ template <typename R> struct Core {
     // strengths): the half spectrum, x frequencies k1 <= 0 (h1x of them)
     // and y, z frequencies -N/2 .. N - N/2 (one more than kept, so that
     // every mode's mirror is among them); the final pass writes the mirrors.
-    int64_t h1x(bool herm) const { return herm ? N[0] / 2 + 1 : N[0]; }
+    // (rows of the half spectrum padded to 16 columns: 128-byte FP32 row
+    // segments of the y pass start on cache lines)
+    int64_t h1v(bool herm) const { return herm ? N[0] / 2 + 1 : N[0]; }
+    int64_t h1x(bool herm) const { return herm ? (h1v(true) + 15) / 16 * 16 : N[0]; }
     ColPass colpass(int axis, bool final, int nsp, bool herm = false) const
     {
         ColPass c{};
@@ -574,6 +577,7 @@ template <typename R> struct Core {
             c.hmid = (dim == 3) ? 1 : 0;
             c.hN1 = (int)N[0];
             c.hH1 = (int)H1;
+            c.hH1v = (int)h1v(true);
             c.hN2 = (dim == 3) ? (int)N[1] : 1;
             c.hsa = (dim == 3) ? N[0] * N[1] : N[0];
             c.dH1 = make_fdiv((uint32_t)H1);
@@ -656,6 +660,7 @@ template <typename R> struct Core {
         J.dby = make_fdiv((uint32_t)by);
         J.lds_tw = (int)spread_lds_twiddles<R>(by * bz, g.n1, g.w);
         J.g1_len = g.N1;
+        J.g1_keep = g.N1;
         return J;
     }
     // estimated cost of one tile of this geometry with M points per transform
@@ -1439,6 +1444,7 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
             J.amax_s = wbound[q];
             J.tmajor = merged[q] ? 1 : 0;
             J.g1_len = (int)c.h1x(herm[q]);
+            J.g1_keep = (int)c.h1v(herm[q]);
             J.tile0 = tile;
             tile += J.ntiles * ntr;
             sjobs[wi].push_back(J);

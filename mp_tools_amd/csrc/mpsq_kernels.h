@@ -76,11 +76,12 @@ struct ColPass {
     // among the kept ones.
     int NO;
     // herm (final pass): the columns are the half spectrum of a transform of
-    // real strengths, x frequencies kh in [0, hH1) (k1 = kh - N1/2 <= 0) and,
+    // real strengths, x frequencies kh in [0, hH1v) (k1 = kh - N1/2 <= 0;
+    // rows padded to hH1 columns, whose pad columns are not written) and,
     // with hmid, middle (y) indices in [0, hN2 + 1); the pass writes each
     // kept mode and its mirror F(-k) = conj F(k) into the full [N][hN2][hN1]
     // map (hsa: the map stride of the transformed axis)
-    int herm, hmid, hN1, hH1, hN2;
+    int herm, hmid, hN1, hH1, hH1v, hN2;
     int64_t hsa;
     FDiv dH1;
 };
@@ -139,7 +140,8 @@ template <typename R> struct SpreadJob {
     int tmajor;                          // tile ids transform-major (else position-major)
     FDiv dntr, dnty, dby, dnt;           // ntr, tiles along y, by, ntiles
     int lds_tw;                          // byte offset of the twiddles in dynamic LDS
-    int g1_len;                          // G1 row length: N1, or N1 / 2 + 1 (half spectrum, real strengths)
+    int g1_len;                          // G1 row length: N1, or the half spectrum's padded to 16
+    int g1_keep;                         // kept frequencies per row: N1, or N1 / 2 + 1 (k1 <= 0)
 };
 
 // one column-FFT pass of one job: blocks [blk0, blk0 + ncb*outer*ngroups)

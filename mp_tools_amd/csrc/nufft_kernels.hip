@@ -585,6 +585,7 @@ __global__ __launch_bounds__(NT) void k_colfft(const ColJob<R> *__restrict__ job
                 }
                 C x = acc[i];
                 if (cp.conj) x.y = -x.y;
+                if (kh >= cp.hH1v) continue;  // pad column
                 if (kk < N && km < N2) dst[(int64_t)kk * cp.hsa + (int64_t)km * N1 + kh] = x;
                 const int ka = 2 * h - kk, kmm = cp.hmid ? 2 * h2 - km : 0, kx = 2 * h1 - kh;
                 if (kh < h1 && kx < N1 && ka >= 0 && ka < N && kmm >= 0 && kmm < N2)

@@ -530,7 +530,7 @@ __device__ __forceinline__ void store_kept(const SpreadJob<R> &J, int t, int y0,
     if (f < N1 - h1) kk = f + h1;
     else if (f >= n1 - h1) kk = f - (n1 - h1);
     else return;
-    if (kk >= J.g1_len) return;  // half spectrum: k1 <= 0 only
+    if (kk >= J.g1_keep) return;  // half spectrum: k1 <= 0 only
     v = cscale(v, J.phinv1[kk]);
     if (e2) v = cmk<typename cplx<R>::T>(ldexp(v.x, e2), ldexp(v.y, e2));
     if (J.final_conj) v.y = -v.y;
@@ -806,7 +806,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 auto store_tile = [&](int row, int i, C v) { tile[__mul24(row, RSX) + padx<true>(i)] = v; };
                 constexpr int NW = NT / 64;
                 const int pairs = rows >> 1;
-                const int KO = J.g1_len;  // kept outputs per row (N1, or the half spectrum k1 <= 0)
+                const int KO = J.g1_keep;  // kept outputs per row (N1, or the half spectrum k1 <= 0)
                 const bool cj = J.final_conj != 0;
                 C *G1 = J.G1;
                 // kept frequency kk of rows 2 pr and 2 pr + 1 from Z = FFT(row 2 pr + i row 2 pr + 1)
@@ -858,7 +858,7 @@ __global__ __launch_bounds__(NarrowCfg::NT, 4) void k_spread_narrow(const Spread
                 if (f < N1 - h1) kk = f + h1;
                 else if (f >= n1 - h1) kk = f - (n1 - h1);
                 else return;
-                if (kk >= J.g1_len) return;
+                if (kk >= J.g1_keep) return;
                 const float sc = phs[kk];
                 gstore(G1 + rowoff[row] + kk, cmk<C>(v.x * sc, cj ? -(v.y * sc) : v.y * sc));
             };
