@@ -204,11 +204,15 @@ def main():
     n_fine = (info32["n1"], info32["n2"], info32["n3"])
     s_c = 8
     # scalar species weights: a frame is one transform over all atoms with
-    # strengths w_s c_j, so every pass after the spread moves one plane per frame
+    # strengths w_s c_j, so every pass after the spread moves one plane per frame;
+    # q_center 0: the strengths are real, and the x and y passes carry the
+    # half spectrum (N1/2 + 1 x frequencies, N2 + 1 y frequencies) whose
+    # mirrors the final pass writes
+    H1, N2o = N[0] // 2 + 1, N[1] + 1
     alg = {
-        "spread_xfft": T * (M * (12 + s_c) + s_c * n_fine[2] * n_fine[1] * N[0]),
-        "colfft": T * s_c * (n_fine[2] * n_fine[1] * N[0] + n_fine[2] * N[1] * N[0]),
-        "final_colfft": T * s_c * (n_fine[2] * N[1] * N[0] + NQ),
+        "spread_xfft": T * (M * (12 + s_c) + s_c * n_fine[2] * n_fine[1] * H1),
+        "colfft": T * s_c * (n_fine[2] * n_fine[1] * H1 + n_fine[2] * N2o * H1),
+        "final_colfft": T * s_c * (n_fine[2] * N2o * H1 + NQ),
         "reduce": T * NQ * s_c + NQ * (n_f + 1) * 8,
         "sort": T * M * (4 + 2 * 32 + 2 * s_c),
         "stage": T * M * (16 + 32 + 4 + s_c),

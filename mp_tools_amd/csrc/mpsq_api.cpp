@@ -1076,11 +1076,17 @@ template <typename R> struct Engine {
             at_ind.ensure(sizeof(int) * 4 * (size_t)n_tot);
             HIPCHK(hipMemcpy(at_ind.p, s->at_ind, sizeof(int) * 4 * (size_t)n_tot, hipMemcpyHostToDevice));
         }
-        // batch size: intermediate buffers of about 2 GiB per batch, <= 16 frames
+        // batch size: the caller's, else intermediate buffers of up to 16 GiB
+        // (at most 1/16 of the device's memory), <= 128 frames: fewer, longer
+        // launches shorten the tails of the persistent passes (C2 on MI355X:
+        // 8060 frames/s at 16 frames per launch, 8450 at 64, 8520 at 128)
         const int tpf = (combine && !wtab.p) ? 1 : nact;  // transforms per frame when run alone
-        int b = s->batch_frames > 0 ? s->batch_frames
-                                    : (int)std::max<size_t>(1, ((size_t)2 << 30) / bytes_per_frame(tpf));
-        B = std::max(1, std::min(b, 16));
+        size_t tot = 0;
+        if (hipDeviceTotalMem(&tot, core.device) != hipSuccess || tot == 0) tot = (size_t)32 << 30;
+        const size_t budget = std::min((size_t)16 << 30, tot / 16);
+        int b = s->batch_frames > 0 ? std::min(s->batch_frames, 1024)
+                                    : (int)std::min<size_t>(std::max<size_t>(1, budget / bytes_per_frame(tpf)), 128);
+        B = std::max(1, b);
     }
 
     void amplitudes(const float *d_pos, int64_t T, void *d_amp)
