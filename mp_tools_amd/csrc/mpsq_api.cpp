@@ -459,7 +459,13 @@ template <typename R> struct Core {
         // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
         // one tile contiguous
         real = real_ && sizeof(R) == 4;
-        rows_max = spread_max_rows<R>((int)n[0]) * (real ? 2 : 1);
+        // real strengths: up to 32 rows where the complex kernel's registers
+        // stop short of that (n1 = 512: 16 -> 32); grids that already allow
+        // 32 rows or more keep their cap, since taller tiles of dense inputs
+        // overflow the narrow kernel's covering bound and every wide pass
+        // re-walks the whole tile's candidates
+        rows_max = spread_max_rows<R>((int)n[0]);
+        if (real && rows_max < 32) rows_max = std::min(2 * rows_max, 32);
         const int bmax = spread_max_tile_b();
         int by = 1, bz = 1;
         if (dim == 2) {
