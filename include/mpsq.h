@@ -157,7 +157,7 @@ typedef struct mpsq_sqw_spec {
     float pmin[3], pmax[3];   /* at_pos_min / at_pos_max */
     float margin;             /* BULK margin (eps_x = 0.1 for one box, else 1) */
     const int32_t *at_ind;    /* host [n_tot][4] cell indices for select_mode 1, or NULL */
-    int batch_frames;         /* frames per launch batch (0 = auto) */
+    int batch_frames;         /* frames per launch batch (0 = auto: <= 128 frames, <= 16 GiB of intermediates) */
     mpsq_opts opts;
 } mpsq_sqw_spec;
 
