@@ -408,6 +408,9 @@ template <typename R> struct Core {
     // FP32 with real strengths: the narrow spread kernel's real form, whose
     // 32-bit cells and paired-row x FFT allow tiles of twice the rows
     bool real = false;
+    // real strengths (either precision): the half spectrum through the
+    // column passes
+    bool real_str = false;
     int rows_max = 1;
 
     void setup(int dim_, const int64_t *nm, int iflag_, double eps_, const mpsq_opts *o, bool real_ = false)
@@ -458,6 +461,7 @@ template <typename R> struct Core {
         // spread tile: whole x extent x (by x bz) rows within the LDS tile;
         // by + w <= n2 and bz + w <= n3 keep the rows a kernel stencil has in
         // one tile contiguous
+        real_str = real_;
         real = real_ && sizeof(R) == 4;
         // real strengths: up to 32 rows where the complex kernel's registers
         // stop short of that (n1 = 512: 16 -> 32); grids that already allow
@@ -1320,7 +1324,9 @@ static void run_parts(const std::vector<Part<R>> &parts, Workspace<R> &ws, Arena
         const Core<R> &c = parts[q].e->core;
         const int rows = ((c.dim >= 2) ? c.g.by : 1) * ((c.dim >= 3) ? c.g.bz : 1);
         real_ok[q] = c.real && spread_real_ok(rows, c.g.n1, c.g.w);
-        herm[q] = real_ok[q] && !part_wtab(q);
+        // FP32: only jobs of the real narrow form (the complex narrow kernel
+        // writes whole rows); FP64: the wide kernel keeps the half itself
+        herm[q] = (sizeof(R) == 8 ? c.real_str : real_ok[q]) && !part_wtab(q);
     }
     auto tpf = [&](int q) { return merged[q] ? 1 : parts[q].e->nact; };  // transforms per frame
     for (int q = 0; q < np; ++q) {

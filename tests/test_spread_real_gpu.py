@@ -36,10 +36,13 @@ def _oracle(pos, counts, nm, ncell, window):
 
 
 # (200, 24, 8): n1 = 400 leaves each row pair's accumulators inside its FFT
-# slot (the wave-level FFT); the smaller grids take the block-wide one
+# slot (the wave-level FFT); the smaller grids take the block-wide one.  The
+# FP64 form has no real spread kernel but runs the half spectrum through its
+# column passes as well.
+@pytest.mark.parametrize("precision,eps,tol", [("fp32", 1e-6, 2e-5), ("fp64", 1e-10, 1e-8)])
 @pytest.mark.parametrize("nm", [(200, 24, 8), (16, 12, 8), (15, 13, 9), (24, 20)])
 @pytest.mark.parametrize("window", [False, True])
-def test_real_strengths_vs_oracle(gpu_lib, nm, window):
+def test_real_strengths_vs_oracle(gpu_lib, precision, eps, tol, nm, window):
     import torch
     from mp_tools_amd.sqw import SqwEngine
     ncell, T = 5, 4
@@ -47,16 +50,16 @@ def test_real_strengths_vs_oracle(gpu_lib, nm, window):
     w = np.array([0.507, 0.716, 0.580])
     F = _oracle(pos, counts, nm, ncell, window)
     dpos = torch.tensor(pos, device="cuda")
-    eng = SqwEngine(len(nm), nm, counts, eps=1e-6, precision="fp32", weights=w,
+    eng = SqwEngine(len(nm), nm, counts, eps=eps, precision=precision, weights=w,
                     tpe=(2 * np.pi / ncell,) * 3, window=window)
     amp = eng.amplitudes(dpos)
     torch.cuda.synchronize()
-    assert rel_l2(amp.cpu().numpy(), (F * w[None, :, None]).sum(axis=1)) < 2e-5
-    eng2 = SqwEngine(len(nm), nm, counts, eps=1e-6, precision="fp32", mask=[1, 0, 1], combine=False,
+    assert rel_l2(amp.cpu().numpy(), (F * w[None, :, None]).sum(axis=1)) < tol
+    eng2 = SqwEngine(len(nm), nm, counts, eps=eps, precision=precision, mask=[1, 0, 1], combine=False,
                      tpe=(2 * np.pi / ncell,) * 3, window=window)
     a2 = eng2.amplitudes(dpos)
     torch.cuda.synchronize()
-    assert rel_l2(a2.cpu().numpy(), F[:, [0, 2]]) < 2e-5
+    assert rel_l2(a2.cpu().numpy(), F[:, [0, 2]]) < tol
 
 
 @pytest.mark.parametrize("nm", [(256, 16, 8), (20, 16, 12)])
