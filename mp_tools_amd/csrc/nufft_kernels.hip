@@ -178,9 +178,12 @@ __global__ __launch_bounds__(256) void k_stage_traj(const StageJob *__restrict__
         }
     double cr = 0.0, ci = 0.0;
     if (keep) {
-        const double ph = 6.283185307179586476925286766559 * ((sp.q0[0] * r0 + sp.q0[1] * r1) + sp.q0[2] * r2);
-        double sn, cs;
-        sincos(ph, &sn, &cs);
+        double sn = 0.0, cs = 1.0;  // q0 = 0 (uniform): the phase is 0 for every finite position
+        if (sp.q0[0] != 0.0 || sp.q0[1] != 0.0 || sp.q0[2] != 0.0) {
+            const double ph =
+                6.283185307179586476925286766559 * ((sp.q0[0] * r0 + sp.q0[1] * r1) + sp.q0[2] * r2);
+            sincos(ph, &sn, &cs);
+        }
         cr = cs;
         ci = -sn;
         if (sp.window) {
