@@ -478,8 +478,13 @@ template <typename R> struct Core {
             double best = 1e300;
             for (int b1 = 1; b1 <= rows_max && b1 <= bmax && n[1] % b1 == 0 && b1 + w <= n[1]; b1 *= 2)
                 for (int b2 = 1; b1 * b2 <= rows_max && b2 <= bmax && n[2] % b2 == 0 && b2 + w <= n[2]; b2 *= 2) {
+                    // (ties: more rows, then the taller y extent, whose
+                    // covering bounds, summed over fewer z-bin rows, are
+                    // smaller: C2's real 8 x 4 tiles give 1.51e-5 at random
+                    // Q against 1.58e-5 for 4 x 8, at the same speed)
                     const double cost = (double)(b1 + w - 1) * (b2 + w - 1) / (b1 * b2);
-                    if (cost < best - 1e-12 || (std::fabs(cost - best) < 1e-12 && b1 * b2 > by * bz)) {
+                    const bool tie = std::fabs(cost - best) < 1e-12;
+                    if (cost < best - 1e-12 || (tie && b1 * b2 > by * bz) || (tie && b1 * b2 == by * bz && b1 > by)) {
                         best = cost;
                         by = b1;
                         bz = b2;
