@@ -1761,8 +1761,9 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    double best = 1e300;
+    double best = 1e300, t_heur = 1e300;
     int bb[2] = {c.g.by, c.g.bz};
+    const int hb[2] = {c.g.by, c.g.bz};  // the shape the engine takes untuned (heuristic or earlier tuning)
     for (int by = 1; by <= (dim >= 2 ? bmax : 1); by *= 2)
         for (int bz = 1; bz <= (dim >= 3 ? bmax : 1); bz *= 2) {
             if (by * bz > c.rows_max) continue;
@@ -1781,12 +1782,21 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
             HIPCHK(hipEventSynchronize(e1));
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (by == hb[0] && bz == hb[1]) t_heur = t;
             if (t < best) {
                 best = t;
                 bb[0] = by;
                 bb[1] = bz;
             }
         }
+    // within 2% of the fastest that shape stays (timing noise; the
+    // heuristic's tie-breaks favour smaller covering bounds, i.e. a finer
+    // FP32 fixed-point scale)
+    if (t_heur <= 1.02 * best) {
+        bb[0] = hb[0];
+        bb[1] = hb[1];
+        best = t_heur;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     {
