@@ -503,7 +503,7 @@ template <typename R> struct Core {
         if (o->tile_by > 0 && dim >= 2 && n[1] % o->tile_by == 0) by = o->tile_by;
         if (o->tile_bz > 0 && dim >= 3 && n[2] % o->tile_bz == 0) bz = o->tile_bz;
         if (by * bz > rows_max || by > bmax || bz > bmax) throw Err{MPSQ_ERR_BAD_ARG};
-        if (spread_lds<R>(by * bz, (int)n[0], w) > 156 * 1024) throw Err{MPSQ_ERR_SIZE};
+        if (spread_lds<R>(by * bz, (int)n[0], w, real) > 156 * 1024) throw Err{MPSQ_ERR_SIZE};
         if ((dim >= 2 && by + w > n[1]) || (dim >= 3 && bz + w > n[2])) throw Err{MPSQ_ERR_BAD_ARG};
         g.by = by;
         g.bz = bz;
@@ -639,7 +639,7 @@ template <typename R> struct Core {
     size_t spread_lds_size() const
     {
         const int by = (dim >= 2) ? g.by : 1, bz = (dim >= 3) ? g.bz : 1;
-        return spread_lds<R>(by * bz, g.n1, g.w);
+        return spread_lds<R>(by * bz, g.n1, g.w, real);
     }
     SpreadJob<R> spreadjob(int ntr, const PtRec<R> *srec, int64_t nrec, int *err, const Cx<R> *cs, int64_t c_tstride,
                            const int64_t *bstart, const int *bmax, int bin_tstride, Cx<R> *G1, const float *amax,
@@ -673,7 +673,7 @@ template <typename R> struct Core {
         J.tmajor = 0;
         J.dnty = make_fdiv((uint32_t)((dim >= 2) ? g.n2 / by : 1));
         J.dby = make_fdiv((uint32_t)by);
-        J.lds_tw = (int)spread_lds_twiddles<R>(by * bz, g.n1, g.w);
+        J.lds_tw = (int)spread_lds_twiddles<R>(by * bz, g.n1, g.w, real);
         J.g1_len = g.N1;
         J.g1_keep = g.N1;
         return J;
@@ -1769,7 +1769,7 @@ static int autotune_impl(const mpsq_sqw_spec *spec, const float *d_pos, int64_t 
             if (by * bz > c.rows_max) continue;
             if (dim >= 2 && (n2 % by || by + w > n2)) continue;
             if (dim >= 3 && (n3 % bz || bz + w > n3)) continue;
-            if (spread_lds<R>(by * bz, n1, w) > 156 * 1024) continue;
+            if (spread_lds<R>(by * bz, n1, w, c.real) > 156 * 1024) continue;
             mpsq_sqw_spec sj = *spec;
             sj.opts.tile_by = by;
             sj.opts.tile_bz = bz;

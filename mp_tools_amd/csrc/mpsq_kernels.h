@@ -204,9 +204,10 @@ int spread_max_wide_passes();
 // and the rows one x-FFT group of such a tile holds
 template <typename R> int spread_max_rows(int n1);
 template <typename R> int spread_group_rows(int rows, int n1);
-template <typename R> size_t spread_lds(int rows, int n1, int w);
+// (real: the geometry of a real-strength plan, whose narrow tiles have 4-byte cells)
+template <typename R> size_t spread_lds(int rows, int n1, int w, bool real = false);
 // byte offset of the twiddles in a spread workgroup's dynamic LDS
-template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w);
+template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w, bool real = false);
 // FP32 tiles of real strengths: whether a geometry can take the real narrow
 // kernel (rows in pairs, one 32-bit cell per row cell, two rows per complex
 // x FFT) and that kernel's dynamic LDS

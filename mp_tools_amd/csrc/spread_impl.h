@@ -162,21 +162,22 @@ template <typename R> int spread_group_rows(int rows, int n1)
     return g;
 }
 // bytes of the tile region (accumulators / FFT rows); the twiddles follow it
+// (real: the narrow kernel's real form, 4-byte cells)
 __host__ __device__ constexpr int narrow_bias(int w) { return (8 * w + 15) / 16 * 16; }
-template <typename R> size_t spread_tile_bytes(int rows, int n1, int w)
+template <typename R> size_t spread_tile_bytes(int rows, int n1, int w, bool real = false)
 {
     const size_t G = (size_t)spread_group_rows<R>(rows, n1);
     const size_t RSX = (size_t)spread_row_stride_x(n1, w);
-    const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * 8 + (size_t)narrow_bias(w) : 0;
+    const size_t narrowb = sizeof(R) == 4 ? (size_t)rows * RSX * (real ? 4 : 8) + (size_t)narrow_bias(w) : 0;
     const size_t wideb = std::max(2 * G * spread_row_stride(n1, w) * 8, G * RSX * sizeof(typename cplx<R>::T));
     return (std::max(narrowb, wideb) + 15) / 16 * 16;
 }
-template <typename R> size_t spread_lds_bytes(int rows, int n1, int w)
+template <typename R> size_t spread_lds_bytes(int rows, int n1, int w, bool real = false)
 {
     // + n1 twiddles; FP32: + the tile's scaled deconvolution factors (N1 <= n1 / 2
     // floats) and its rows' output offsets (rows int64)
     const size_t extra = sizeof(R) == 4 ? (size_t)(n1 / 2) * 4 + (size_t)rows * 8 : 0;
-    return spread_tile_bytes<R>(rows, n1, w) + (size_t)n1 * sizeof(typename cplx<R>::T) + extra;
+    return spread_tile_bytes<R>(rows, n1, w, real) + (size_t)n1 * sizeof(typename cplx<R>::T) + extra;
 }
 // byte offset of the narrow kernel's per-tile tables after the twiddles
 __host__ __device__ inline int spread_tab_offset(int lds_tw, int n1) { return lds_tw + n1 * 8; }

@@ -31,8 +31,14 @@ template <typename R> int spread_tile_elems() { return SpreadCfg<R>::E * SpreadC
 int spread_max_runs() { return MAX_RUNS; }
 int spread_max_tile_b() { return MAX_TILE_B; }
 int spread_max_wide_passes() { return MAX_WIDE_PASSES; }
-template <typename R> size_t spread_lds(int rows, int n1, int w) { return spread_lds_bytes<R>(rows, n1, w); }
-template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w) { return spread_tile_bytes<R>(rows, n1, w); }
+template <typename R> size_t spread_lds(int rows, int n1, int w, bool real)
+{
+    return spread_lds_bytes<R>(rows, n1, w, real);
+}
+template <typename R> size_t spread_lds_twiddles(int rows, int n1, int w, bool real)
+{
+    return spread_tile_bytes<R>(rows, n1, w, real);
+}
 
 __global__ void k_zero_i32(int *__restrict__ p, int n)
 {
@@ -77,8 +83,8 @@ void launch_spread_jobs(const SpreadJob<R> *jobs, const int *tile0s, int njobs, 
     template int spread_tile_elems<R>();                                                                              \
     template int spread_max_rows<R>(int);                                                                             \
     template int spread_group_rows<R>(int, int);                                                                      \
-    template size_t spread_lds<R>(int, int, int);                                                                     \
-    template size_t spread_lds_twiddles<R>(int, int, int);                                                            \
+    template size_t spread_lds<R>(int, int, int, bool);                                                               \
+    template size_t spread_lds_twiddles<R>(int, int, int, bool);                                                      \
     template void launch_spread_jobs<R>(const SpreadJob<R> *, const int *, int, int, const int *, int *, int *,      \
                                         size_t, hipStream_t, size_t);
 

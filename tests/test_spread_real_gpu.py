@@ -99,3 +99,22 @@ def test_real_dense_tiles_take_the_wide_kernel(gpu_lib):
     torch.cuda.synchronize()
     F = _oracle(pos, counts, nm, ncell, False)
     assert rel_l2(amp.cpu().numpy(), (F * w[None, :, None]).sum(axis=1)) < 2e-5
+
+
+@pytest.mark.parametrize("precision,eps,tol", [("fp32", 1e-6, 2e-5), ("fp64", 1e-10, 1e-8)])
+def test_real_long_rows(gpu_lib, precision, eps, tol):
+    """N1 = 1024 (fine rows of 2048 cells): a real-strength plan doubles the
+    complex form's 4-row tiles to 8 rows of 4-byte cells, whose LDS (tile,
+    2048 twiddles, tables) fits where the complex form's 8-byte cells would
+    not."""
+    import torch
+    from mp_tools_amd.sqw import SqwEngine
+    ncell, T = 4, 2
+    pos, counts = _traj(ncell, T, seed=11)
+    nm = (1024, 12, 8)
+    w = np.array([0.507, 0.716, 0.580])
+    eng = SqwEngine(3, nm, counts, eps=eps, precision=precision, weights=w, tpe=(2 * np.pi / ncell,) * 3)
+    amp = eng.amplitudes(torch.tensor(pos, device="cuda"))
+    torch.cuda.synchronize()
+    F = _oracle(pos, counts, nm, ncell, False)
+    assert rel_l2(amp.cpu().numpy(), (F * w[None, :, None]).sum(axis=1)) < tol
