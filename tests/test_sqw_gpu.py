@@ -8,7 +8,7 @@ E(Q) branch :1480-1517 gives the same |X|) with the normalisation of
 :1527 / :1576.
 
 Tolerances (relative l2): FP64 form at eps 1e-10: 1e-8; FP32 form at
-eps 1e-6: 1e-5 on S(Q,w), 2e-5 on amplitudes.
+eps 1e-6: 1e-5 on S(Q,w) (the accuracy bar of the FP32 form), 2e-5 on amplitudes.
 """
 import numpy as np
 import pytest
