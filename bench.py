@@ -233,6 +233,12 @@ def main():
     res["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                        "algorithmic_bytes_per_launch": per_launch_bytes}
+    if dom == "spread_xfft":
+        # the same launch priced with the complex form's output (all N1 x
+        # frequencies), the byte model of a transform of complex strengths
+        full = T * (M * (12 + s_c) + s_c * n_fine[2] * n_fine[1] * N[0]) / launches
+        res["roofline"]["frac_full_spectrum_bytes"] = (
+            full / (stages[dom]["avg_ms_per_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
     # the spread kernel is bound by VALU issue, not HBM: its measured share of
     # the SIMDs' VALU issue slots (SQ_INSTS_VALU x 4 cycles / (SIMDs x cycles))
     cfile = os.path.join(ROOT, "profiles", "c2_fp32_instruction_counters.txt")
